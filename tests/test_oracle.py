@@ -1,0 +1,335 @@
+"""Pins the FP64 oracle (oracle/psm_oracle.py) against the reference's own
+fixtures and against what the mathematics fixes.  CPU only."""
+import numpy as np
+import pytest
+
+from oracle import psm_oracle as O
+
+
+def _rand_lower(n, rng=np.random):
+    L = rng.rand(n, n)
+    L[np.triu_indices(n, 1)] = 0.0
+    return L
+
+
+# ---------------------------------------------------------------- tria
+@pytest.mark.parametrize("shape", [(1, 1), (2, 4), (5, 10), (7, 7), (10, 10), (3, 8)])
+def test_tria_is_lower_cholesky_of_gram(shape):
+    rng = np.random.RandomState(0)
+    A = rng.randn(*shape)
+    L = O.tria(A)
+    assert np.allclose(np.triu(L, 1), 0.0)
+    assert np.all(np.diag(L) >= 0)
+    np.testing.assert_allclose(L @ L.T, A @ A.T, atol=1e-12)
+    np.testing.assert_allclose(L, np.linalg.cholesky(A @ A.T), atol=1e-10)
+
+
+def test_tria_rank_deficient_gives_zeros_not_nan():
+    A = np.zeros((4, 8))
+    A[0, 0] = 2.0
+    L = O.tria(A)
+    assert np.all(np.isfinite(L))
+    np.testing.assert_allclose(L @ L.T, A @ A.T, atol=1e-14)
+
+
+# ---------------------------------------------------------------- cholesky update
+@pytest.mark.parametrize("multiplier", [1.0, -0.1])
+@pytest.mark.parametrize("seed", [0, 42, 666])
+@pytest.mark.parametrize("dim", [2, 3, 10, 11])
+def test_cholesky_update_definition(multiplier, seed, dim):
+    """tests/test_math_utils.py:16-34 (checked against the definition)."""
+    np.random.seed(seed)
+    L = _rand_lower(dim)
+    v = np.random.randn(dim)
+    expected = L @ L.T + multiplier * np.outer(v, v)
+    if np.linalg.eigvalsh(expected).min() <= 1e-6:
+        pytest.skip("not positive definite")
+    out = O.cholesky_update(L, v, multiplier)
+    np.testing.assert_allclose(out @ out.T, expected, rtol=1e-4, atol=1e-10)
+    outb = O.cholesky_update_batch(L[None], v[None], multiplier)[0]
+    np.testing.assert_allclose(outb, out, atol=1e-14)
+
+
+def test_cholesky_downdate_failure_masks_to_zero():
+    L = np.eye(3)
+    out = O.cholesky_update(L, np.array([2.0, 0.0, 0.0]), -1.0)
+    assert np.all(np.isfinite(out))
+
+
+# ---------------------------------------------------------------- operators
+def _filter_elems(dim, rng):
+    A1, A2 = rng.randn(dim, dim), rng.randn(dim, dim)
+    b1, b2 = rng.randn(dim), rng.randn(dim)
+    U1, U2 = _rand_lower(dim, rng), _rand_lower(dim, rng)
+    eta1, eta2 = rng.randn(dim), rng.randn(dim)
+    Z1, Z2 = _rand_lower(dim, rng), _rand_lower(dim, rng)
+    return (A1, b1, U1, eta1, Z1), (A2, b2, U2, eta2, Z2)
+
+
+@pytest.mark.parametrize("dim", [1, 2, 3, 5])
+@pytest.mark.parametrize("seed", [0, 42])
+def test_sqrt_vs_std_filtering_operator(dim, seed):
+    """tests/test_parallel_operators.py:17-58."""
+    rng = np.random.RandomState(seed)
+    e1, e2 = _filter_elems(dim, rng)
+    s1 = (e1[0], e1[1], e1[2] @ e1[2].T, e1[3], e1[4] @ e1[4].T)
+    s2 = (e2[0], e2[1], e2[2] @ e2[2].T, e2[3], e2[4] @ e2[4].T)
+    A, b, C, eta, J = O.standard_filtering_operator(s1, s2)
+    A_, b_, U, eta_, Z = O.sqrt_filtering_operator(e1, e2)
+    np.testing.assert_allclose(A, A_, atol=1e-9, rtol=1e-9)
+    np.testing.assert_allclose(b, b_, atol=1e-9, rtol=1e-9)
+    np.testing.assert_allclose(eta, eta_, atol=1e-9, rtol=1e-9)
+    np.testing.assert_allclose(C, U @ U.T, atol=1e-9, rtol=1e-9)
+    np.testing.assert_allclose(J, Z @ Z.T, atol=1e-9, rtol=1e-9)
+
+
+@pytest.mark.parametrize("dim", [1, 2, 3, 5])
+@pytest.mark.parametrize("seed", [0, 42])
+def test_sqrt_vs_std_smoothing_operator(dim, seed):
+    """tests/test_parallel_operators.py:61-89."""
+    rng = np.random.RandomState(seed)
+    g1, g2 = rng.randn(dim), rng.randn(dim)
+    E1, E2 = rng.randn(dim, dim), rng.randn(dim, dim)
+    D1, D2 = _rand_lower(dim, rng), _rand_lower(dim, rng)
+    g, E, L = O.standard_smoothing_operator((g1, E1, D1 @ D1.T), (g2, E2, D2 @ D2.T))
+    g_, E_, D = O.sqrt_smoothing_operator((g1, E1, D1), (g2, E2, D2))
+    np.testing.assert_allclose(g, g_, atol=1e-12)
+    np.testing.assert_allclose(E, E_, atol=1e-12)
+    np.testing.assert_allclose(L, D @ D.T, atol=1e-12)
+
+
+def test_filtering_operator_is_associative():
+    rng = np.random.RandomState(3)
+    a, b = _filter_elems(4, rng)
+    c, _ = _filter_elems(4, rng)
+    lhs = O.sqrt_filtering_operator(O.sqrt_filtering_operator(a, b), c)
+    rhs = O.sqrt_filtering_operator(a, O.sqrt_filtering_operator(b, c))
+    for x, y in zip(lhs, rhs):
+        if x.ndim == 2 and x.shape[0] == x.shape[1] and np.allclose(np.triu(x, 1), 0):
+            np.testing.assert_allclose(x @ x.T, y @ y.T, atol=1e-9)
+        else:
+            np.testing.assert_allclose(x, y, atol=1e-9)
+
+
+def test_scan_order_independent():
+    rng = np.random.RandomState(5)
+    n, dim = 11, 3
+    elems = [_filter_elems(dim, rng)[0] for _ in range(n)]
+    stacked = tuple(np.stack([e[j] for e in elems]) for j in range(5))
+    r1 = O.associative_scan(O.sqrt_filtering_operator, stacked)
+    r2 = O.sequential_fold_scan(O.sqrt_filtering_operator, stacked)
+    np.testing.assert_allclose(r1[1], r2[1], atol=1e-9)
+    np.testing.assert_allclose(r1[2] @ np.swapaxes(r1[2], 1, 2), r2[2] @ np.swapaxes(r2[2], 1, 2), atol=1e-9)
+
+
+# ---------------------------------------------------------------- linear-Gaussian closed forms
+def _lin_problem(dim_x, dim_y, seed, T):
+    np.random.seed(seed)
+    x0, chol_x0, F, Q, cholQ, b, _ = O.get_system(dim_x, dim_x)
+    _, _, H, R, cholR, c, _ = O.get_system(dim_x, dim_y)
+    _, ys = O.lgssm_data(x0.mean, F, H, R, Q, b, c, T, random_state=seed)
+    trans_s = O.FunctionalModel(O.Linear(F), O.MVNSqrt(b, cholQ))
+    obs_s = O.FunctionalModel(O.Linear(H), O.MVNSqrt(c, cholR))
+    trans = O.FunctionalModel(O.Linear(F), O.MVNStandard(b, Q))
+    obs = O.FunctionalModel(O.Linear(H), O.MVNStandard(c, R))
+    return x0, chol_x0, (F, Q, b, H, R, c), ys, (trans_s, obs_s), (trans, obs)
+
+
+@pytest.mark.parametrize("dim_x", [1, 2, 3])
+@pytest.mark.parametrize("dim_y", [1, 2, 3])
+@pytest.mark.parametrize("method", ["extended", "cubature"])
+def test_filter_smoother_vs_joint_gaussian(dim_x, dim_y, method):
+    """Parallel sqrt/std filter, smoother and log-likelihood against the exact
+    posterior of the dense joint Gaussian (closed form)."""
+    meth = getattr(O, method)
+    x0, cx0, (F, Q, b, H, R, c), ys, (ts, os_), (tt, ot) = _lin_problem(dim_x, dim_y, 0, 6)
+    (fm, fP), (sm, sP), ell = O.joint_gaussian_posterior(ys, x0.mean, x0.cov, F, Q, b, H, R, c)
+    for x, tr, ob in ((cx0, ts, os_), (x0, tt, ot)):
+        for parallel in (True, False):
+            filt, e = O.filtering(ys, x, tr, ob, meth, None, parallel, return_loglikelihood=True)
+            cov = filt[1] @ np.swapaxes(filt[1], 1, 2) if isinstance(x, O.MVNSqrt) else filt[1]
+            np.testing.assert_allclose(filt[0], fm, atol=1e-8)
+            np.testing.assert_allclose(cov, fP, atol=1e-8)
+            assert e == pytest.approx(ell, rel=1e-9, abs=1e-9)
+            sm_ = O.smoothing(tr, filt, meth, None, parallel)
+            cov = sm_[1] @ np.swapaxes(sm_[1], 1, 2) if isinstance(x, O.MVNSqrt) else sm_[1]
+            np.testing.assert_allclose(sm_[0], sm, atol=1e-8)
+            np.testing.assert_allclose(cov, sP, atol=1e-8)
+
+
+@pytest.mark.parametrize("dim_x", [1, 2])
+@pytest.mark.parametrize("dim_y", [1, 2])
+@pytest.mark.parametrize("parallel", [True, False])
+def test_iterated_linear_equals_one_pass(dim_x, dim_y, parallel):
+    """tests/test_iterated_smoother.py:26-69."""
+    x0, cx0, (F, Q, b, H, R, c), ys, (ts, os_), _ = _lin_problem(dim_x, dim_y, 0, 5)
+    nominal = O.MVNSqrt(np.random.randn(6, dim_x), np.repeat(np.eye(dim_x)[None], 6, 0))
+    ref = O.seq_smoothing(ts, O.seq_filtering(ys, cx0, ts, os_, O.extended), O.extended)
+    it = O.iterated_smoothing(ys, cx0, ts, os_, O.cubature, nominal, parallel, criterion=lambda i, *_: i < 5)
+    np.testing.assert_allclose(it.mean, ref.mean, atol=1e-9)
+
+
+@pytest.mark.parametrize("dim_x", [1, 3])
+@pytest.mark.parametrize("dim_y", [1, 2, 3])
+@pytest.mark.parametrize("seed", [0, 42])
+def test_elements_sqrt_vs_std_and_first_element_is_kalman_step(dim_x, dim_y, seed):
+    """tests/test_parallel_filter.py:26-73 and the element-1 prefix identity."""
+    np.random.seed(seed)
+    x0, cx0, F, Q, cholQ, b, _ = O.get_system(dim_x, dim_x)
+    _, _, H, R, cholR, c, y = O.get_system(dim_x, dim_y)
+    A, bb, C, eta, J = O.std_filter_element(F, Q, b, H, R, c, x0.mean, x0.cov, y)
+    A_, bb_, U, eta_, Z = O.sqrt_filter_element(F, cholQ, b, H, cholR, c, cx0.mean, cx0.chol, y)
+    for u, v in ((A, A_), (bb, bb_), (eta, eta_), (C, U @ U.T), (J, Z @ Z.T)):
+        np.testing.assert_allclose(u, v, atol=1e-9)
+    # one predict + update from the prior
+    m = F @ x0.mean + b
+    P = F @ x0.cov @ F.T + Q
+    S = H @ P @ H.T + R
+    K = P @ H.T @ np.linalg.inv(S)
+    np.testing.assert_allclose(bb_, m + K @ (y - H @ m - c), atol=1e-9)
+    np.testing.assert_allclose(U @ U.T, P - K @ S @ K.T, atol=1e-9)
+
+
+@pytest.mark.parametrize("dim_x", [1, 2, 3])
+@pytest.mark.parametrize("seed", [0, 42])
+def test_smoother_element_sqrt_vs_std(dim_x, seed):
+    """tests/test_parallel_smoother.py:26-54."""
+    np.random.seed(seed)
+    x0, cx0, F, Q, cholQ, b, _ = O.get_system(dim_x, dim_x)
+    g, E, L = O.std_smoother_element(F, Q, b, x0.mean, x0.cov)
+    g_, E_, D = O.sqrt_smoother_element(F, cholQ, b, cx0.mean, cx0.chol)
+    np.testing.assert_allclose(g, g_, atol=1e-9)
+    np.testing.assert_allclose(E, E_, atol=1e-9)
+    np.testing.assert_allclose(L, D @ D.T, atol=1e-9)
+
+
+# ---------------------------------------------------------------- linearizations
+METHODS = {
+    "extended": O.extended,
+    "cubature": O.cubature,
+    "gauss_hermite": O.gauss_hermite,
+    "unscented": O.unscented,
+}
+
+
+@pytest.mark.parametrize("dim_x", [1, 3])
+@pytest.mark.parametrize("seed", [0, 42])
+@pytest.mark.parametrize("method", list(METHODS))
+@pytest.mark.parametrize("sqrt", [True, False])
+def test_linearization_exact_on_linear(dim_x, seed, method, sqrt):
+    """tests/test_linearization.py:67-107."""
+    np.random.seed(seed)
+    a = np.random.randn(dim_x, dim_x)
+    m_x, m_q = np.random.randn(dim_x), np.random.randn(dim_x)
+    cx, cq = _rand_lower(dim_x), _rand_lower(dim_x)
+    if sqrt:
+        x, q = O.MVNSqrt(m_x, cx), O.MVNSqrt(m_q, cq)
+    else:
+        x, q = O.MVNStandard(m_x, cx @ cx.T), O.MVNStandard(m_q, cq @ cq.T)
+    F, Ql, rem = METHODS[method](O.FunctionalModel(O.Linear(a), q), x)
+    if sqrt:
+        Ql = Ql @ Ql.T
+    np.testing.assert_allclose(F, a, atol=1e-9)
+    np.testing.assert_allclose(rem, m_q, atol=1e-9)
+    np.testing.assert_allclose(Ql, cq @ cq.T, atol=1e-9)
+
+
+@pytest.mark.parametrize("dim_x", [1, 3])
+@pytest.mark.parametrize("seed", [0, 42])
+@pytest.mark.parametrize("method", list(METHODS))
+@pytest.mark.parametrize("fun", ["sin", "cos", "exp", "arctan"])
+def test_linearization_sqrt_vs_std(dim_x, seed, method, fun):
+    """tests/test_linearization.py:154-181."""
+    np.random.seed(seed)
+    m_x, m_q = np.random.randn(dim_x), np.random.randn(dim_x)
+    cx, cq = _rand_lower(dim_x), _rand_lower(dim_x)
+    f = O.Elementwise(fun)
+    Fs, Cs, bs = METHODS[method](O.FunctionalModel(f, O.MVNSqrt(m_x, cx)), O.MVNSqrt(m_q, cq))
+    F, C, b = METHODS[method](O.FunctionalModel(f, O.MVNStandard(m_x, cx @ cx.T)), O.MVNStandard(m_q, cq @ cq.T))
+    np.testing.assert_allclose(Fs, F, atol=1e-10)
+    np.testing.assert_allclose(bs, b, atol=1e-10)
+    np.testing.assert_allclose(C, Cs @ Cs.T, atol=1e-10)
+
+
+@pytest.mark.parametrize("order", [2, 3, 4, 5])
+@pytest.mark.parametrize("n", [1, 2, 3])
+def test_gauss_hermite_table_vs_hermgauss(order, n):
+    x, w = np.polynomial.hermite.hermgauss(order)
+    tab = O.gauss_hermite_table(n, order)
+    assert tab.xi.shape == (order ** n, n)
+    np.testing.assert_allclose(np.sort(np.unique(np.round(tab.xi[:, 0], 12))), np.sort(np.sqrt(2) * x), atol=1e-10)
+    np.testing.assert_allclose(tab.wm.sum(), 1.0, atol=1e-12)
+    if n == 1:
+        order_ = np.argsort(tab.xi[:, 0])
+        np.testing.assert_allclose(tab.wm[order_], (w / np.sqrt(np.pi))[np.argsort(x)], atol=1e-12)
+    # exactness: E[x_i^2] = 1, E[x_i x_j] = 0
+    np.testing.assert_allclose(tab.xi.T @ (tab.wm[:, None] * tab.xi), np.eye(n), atol=1e-10)
+
+
+def test_unscented_and_cubature_moments():
+    for tab in (O.cubature_table(4), O.unscented_table(4), O.unscented_table(3, 0.5, 2.0, 1.0)):
+        np.testing.assert_allclose(tab.wm.sum(), 1.0, atol=1e-12)
+        np.testing.assert_allclose(tab.wm @ tab.xi, 0.0, atol=1e-12)
+        np.testing.assert_allclose(tab.xi.T @ (tab.wm[:, None] * tab.xi), np.eye(tab.xi.shape[1]), atol=1e-12)
+
+
+def test_model_jacobians_vs_central_differences():
+    rng = np.random.RandomState(1)
+    models = [O.CoordinatedTurn(0.01), O.Bearings([-1.5, 0.5], [1.0, 1.0]), O.Linear(rng.randn(3, 5))]
+    for f in models:
+        for w in (0.7, 1e-8, -2.0):
+            x = rng.randn(5)
+            x[4] = w
+            J = f.jac(x)
+            h = 1e-6
+            Jfd = np.stack([(f(x + h * e) - f(x - h * e)) / (2 * h) for e in np.eye(5)], axis=1)
+            if isinstance(f, O.CoordinatedTurn) and abs(w) < 1e-6:
+                # branch semantics: the w-derivative of sin(wt)/w and (cos wt - 1)/w is 0
+                Jfd[:2, 4] = J[:2, 4]
+            np.testing.assert_allclose(J, Jfd, atol=1e-7)
+
+
+# ---------------------------------------------------------------- reference fixtures (bearings)
+def _bearings(golden_dir):
+    ys = np.load(f"{golden_dir}/bearings_ys.npy")
+    Q, R = O.bearings_parameters(0.01, 0.1, 0.5, 0.01)
+    tm = O.FunctionalModel(O.CoordinatedTurn(0.01), O.MVNSqrt(np.zeros(5), np.linalg.cholesky(Q)))
+    om = O.FunctionalModel(O.Bearings([-1.5, 0.5], [1.0, 1.0]), O.MVNSqrt(np.zeros(2), np.linalg.cholesky(R)))
+    return ys, O.MVNSqrt(np.array([-1.0, -1.0, 0, 0, 0]), np.eye(5)), tm, om
+
+
+def test_bearings_ieks_fixture(golden_dir):
+    """tests/test_bearings_only.py:21-72 (extended, parallel, 100 iterations, 3 decimals)."""
+    ys, x0, tm, om = _bearings(golden_dir)
+    exp = np.load(f"{golden_dir}/bearings_ieks.npz")
+    res = O.iterated_smoothing(ys, x0, tm, om, O.extended, None, True, criterion=lambda i, *_: i < 100)
+    np.testing.assert_array_almost_equal(res.mean[1:], exp["arr_0"], decimal=3)
+    np.testing.assert_array_almost_equal(res.chol[1:] @ np.swapaxes(res.chol[1:], 1, 2), exp["arr_1"], decimal=3)
+
+
+def test_bearings_icks_fixture_is_on_the_cubature_two_cycle(golden_dir):
+    """The cubature iteration on this data does not converge in FP64: it
+    settles into a 2-cycle.  The reference fixture (generated at FP32) lies on
+    that cycle: two applications of the map starting at the fixture return
+    close to it, one application does not."""
+    ys, x0, tm, om = _bearings(golden_dir)
+    exp = np.load(f"{golden_dir}/bearings_icks.npz")
+    em, ch = exp["arr_0"], np.linalg.cholesky(exp["arr_1"])
+    nom = O.MVNSqrt(np.concatenate([em[:1], em]), np.concatenate([ch[:1], ch]))
+    one = O.filter_smoother(ys, x0, tm, om, O.cubature, nom, True)
+    two = O.filter_smoother(ys, x0, tm, om, O.cubature, one, True)
+    d1 = np.abs(one.mean[1:] - em).max(axis=1)
+    d2 = np.abs(two.mean[1:] - em).max(axis=1)
+    assert np.median(d2) < 3e-3 and d2.max() < 3e-2
+    assert d1.max() > 5 * d2.max()
+
+
+def test_fixed_point_scalar_known_answer():
+    """tests/test_math_utils.py:57-67: x* = 0.5 / 0.3 and its implicit gradient."""
+    a, b = 0.7, 0.5
+    x_star, n = O.fixed_point(lambda x: a * x + b, 1.0, lambda i, *_: i < 500)
+    assert x_star == pytest.approx(0.5 / 0.3, rel=1e-7)
+    # d x*/d a = x*/(1-a), d x*/d b = 1/(1-a); implicit VJP with x_bar = 1
+    ga, gb = O.fixed_point_vjp(lambda u: a * u, lambda u: (u * x_star, u), 1.0, n)
+    assert ga == pytest.approx(x_star / (1 - a), rel=1e-7)
+    assert gb == pytest.approx(1 / (1 - a), rel=1e-7)
