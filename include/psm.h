@@ -1,0 +1,181 @@
+/*
+ * psm.h -- C ABI of libparsmooth_hip.so, the MI355X (gfx950) implementation of
+ * parsmooth's parallel square-root filtering / smoothing path.
+ *
+ * Every entry point is stream-ordered on the caller's HIP stream, takes plain
+ * device pointers and sizes, never allocates or frees caller memory, and
+ * returns 0 on success or a negative PSM_E* status (psm_last_error() gives the
+ * message of the last failure on the calling thread).  Scratch memory for the
+ * scans is a caller-supplied workspace sized by psm_workspace_bytes().
+ *
+ * Arrays are C-contiguous and use the reference's shapes: trajectories are
+ * (n_states, nx) means and (n_states, nx, nx) chol factors (square-root form)
+ * or covariances (covariance form); observations are (n_obs, ny).
+ *
+ * A ragged batch of sequences is laid out back to back: sequence s owns
+ * states [state_offsets[s], state_offsets[s+1]) (T_s + 1 states: the prior
+ * state 0 then one state per observation) and observations
+ * [state_offsets[s] - s, state_offsets[s+1] - s - 1).
+ */
+#ifndef PSM_H_
+#define PSM_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* dtypes */
+#define PSM_F32 0
+#define PSM_F64 1
+
+/* forms: MVNSqrt (chol factors) or MVNStandard (covariances) */
+#define PSM_FORM_SQRT 0
+#define PSM_FORM_STD 1
+
+/* built-in model functions (FunctionalModel.function) */
+#define PSM_MODEL_LINEAR 0            /* x -> A x, params = A (nout x nin) */
+#define PSM_MODEL_COORDINATED_TURN 1  /* params = [dt], nin = nout = 5 */
+#define PSM_MODEL_BEARINGS 2          /* params = [s1x, s1y, s2x, s2y], nout = 2 */
+#define PSM_MODEL_SIN 3               /* elementwise, nin = nout */
+#define PSM_MODEL_COS 4
+#define PSM_MODEL_EXP 5
+#define PSM_MODEL_ARCTAN 6
+
+/* status codes */
+#define PSM_OK 0
+#define PSM_E_ARG (-1)
+#define PSM_E_UNSUPPORTED (-2)
+#define PSM_E_HIP (-3)
+#define PSM_E_WORKSPACE (-4)
+
+/* FunctionalModel(function, MVN(q_mean, q_chol_or_cov)); all pointers on device,
+ * in the problem's dtype. */
+typedef struct psm_model {
+  int32_t kind;
+  int32_t nin;
+  int32_t nout;
+  int32_t pad_;
+  const void* params;
+  const void* q_mean;   /* (nout,) */
+  const void* q_chol;   /* (nout, nout): chol in sqrt form, covariance in covariance form */
+} psm_model_t;
+
+/* Unit sigma-point table: points are m + L xi_p.  A NULL table (or npts == 0)
+ * selects the extended (Taylor) linearization. */
+typedef struct psm_sigma {
+  int32_t npts;
+  int32_t pad_;
+  const void* xi;  /* (npts, n) */
+  const void* wm;  /* (npts,) */
+  const void* wc;  /* (npts,) */
+} psm_sigma_t;
+
+/* A ragged batch of sequences sharing dims, models and linearization. */
+typedef struct psm_problem {
+  int32_t dtype;
+  int32_t form;
+  int32_t nx;
+  int32_t ny;
+  int32_t n_seq;
+  int32_t chunk;         /* states per lane in the chunk passes (0: default) */
+  int64_t n_states;      /* total states = sum(T_s + 1) */
+  const int64_t* state_offsets;  /* device, n_seq + 1 */
+} psm_problem_t;
+
+int psm_version(void);
+const char* psm_last_error(void);
+/* 1 if (dtype, form, nx, ny) has compiled kernels. */
+int psm_supported(int32_t dtype, int32_t form, int32_t nx, int32_t ny);
+
+/* ---------------------------------------------------------------------------
+ * Linearization of a FunctionalModel at n points.
+ * Replaces parsmooth/linearization/_extended.py:9-40 (sigma == NULL) and
+ * _cubature.py:10-34 / _gh.py:13-41 / _unscented.py:10-39 through
+ * _sigma_points.py:63-100 (sigma != NULL, table built by the caller).
+ * x_chol is the chol (sqrt form) or covariance (covariance form) of x.
+ * Outputs F (n, nout, nin), C (n, nout, nout), b (n, nout).
+ * ------------------------------------------------------------------------- */
+int psm_linearize(int32_t dtype, int32_t form, const psm_model_t* model, const psm_sigma_t* sigma,
+                  int64_t n, const void* x_mean, const void* x_chol, void* F, void* C, void* b,
+                  void* stream);
+
+/* ---------------------------------------------------------------------------
+ * Filter elements for n steps from the linearized model and a prior (m, L)
+ * per step.  Replaces parsmooth/parallel/_filtering.py:115-146
+ * (_sqrt_associative_params_one) and :78-97 (_standard_associative_params_one).
+ * Inputs: F (n,nx,nx) cq (n,nx,nx) bt (n,nx) H (n,ny,nx) cr (n,ny,ny) c (n,ny)
+ * m (n,nx) L (n,nx,nx) y (n,ny).  Outputs: A b U eta Z (U/Z or C/J in
+ * covariance form).
+ * ------------------------------------------------------------------------- */
+int psm_filter_elements(int32_t dtype, int32_t form, int32_t nx, int32_t ny, int64_t n,
+                        const void* F, const void* cq, const void* bt, const void* H,
+                        const void* cr, const void* c, const void* m, const void* L,
+                        const void* y, void* A, void* b, void* U, void* eta, void* Z,
+                        void* stream);
+
+/* Batched filtering operator out = e1 (+) e2 (e1 earlier).  Replaces
+ * parsmooth/parallel/_operators.py:43-77 (sqrt) and :7-40 (covariance).
+ * e1, e2, out: arrays of 5 device pointers (A, b, U|C, eta, Z|J). */
+int psm_filter_combine(int32_t dtype, int32_t form, int32_t nx, int64_t n, const void* const* e1,
+                       const void* const* e2, void* const* out, void* stream);
+
+/* Smoother elements for n states from the linearized transition and the
+ * filtered (m, L).  Replaces parsmooth/parallel/_smoothing.py:72-85 and :60-69.
+ * Outputs g (n,nx), E (n,nx,nx), D (n,nx,nx) (D: chol or covariance). */
+int psm_smoother_elements(int32_t dtype, int32_t form, int32_t nx, int64_t n, const void* F,
+                          const void* cq, const void* bt, const void* m, const void* L, void* g,
+                          void* E, void* D, void* stream);
+
+/* Batched smoothing operator (e1 = later aggregate, e2 = earlier).  Replaces
+ * parsmooth/parallel/_operators.py:104-125 and :80-101.  3 pointers each. */
+int psm_smoother_combine(int32_t dtype, int32_t form, int32_t nx, int64_t n, const void* const* e1,
+                         const void* const* e2, void* const* out, void* stream);
+
+/* ---------------------------------------------------------------------------
+ * Whole passes over a ragged batch.
+ * ------------------------------------------------------------------------- */
+size_t psm_workspace_bytes(const psm_problem_t* prob);
+
+/* Parallel filter at a nominal trajectory.  Replaces
+ * parsmooth/parallel/_filtering.py:13-61 (linearization at every step, element
+ * construction, associative scan, x0 prepended, log-likelihood).
+ * ys (n_obs,ny); m0 (n_seq,nx); L0 (n_seq,nx,nx); nominal (n_states,...);
+ * out (n_states,...).  ell (n_seq doubles, device) may be NULL. */
+int psm_filter(const psm_problem_t* prob, const psm_model_t* trans, const psm_model_t* obs,
+               const psm_sigma_t* sigma, const void* ys, const void* m0, const void* L0,
+               const void* nom_m, const void* nom_L, void* out_m, void* out_L, double* ell,
+               void* workspace, size_t workspace_bytes, void* stream);
+
+/* Parallel smoother at a nominal trajectory.  Replaces
+ * parsmooth/parallel/_smoothing.py:14-57 (elements, last element (m_T, 0, L_T),
+ * reverse associative scan). */
+int psm_smoother(const psm_problem_t* prob, const psm_model_t* trans, const psm_sigma_t* sigma,
+                 const void* filt_m, const void* filt_L, const void* nom_m, const void* nom_L,
+                 void* out_m, void* out_L, void* workspace, size_t workspace_bytes, void* stream);
+
+/* One iteration of the fixed-point map: filter then smoother at the same
+ * nominal (parsmooth/methods.py:38-47), sharing the transition linearization.
+ * filt_m/filt_L receive the filtered trajectory.  use_graph != 0 captures the
+ * launch sequence of this argument set in a hipGraph on first use and replays
+ * it afterwards. */
+int psm_filter_smoother(const psm_problem_t* prob, const psm_model_t* trans,
+                        const psm_model_t* obs, const psm_sigma_t* sigma, const void* ys,
+                        const void* m0, const void* L0, const void* nom_m, const void* nom_L,
+                        void* filt_m, void* filt_L, void* out_m, void* out_L, void* workspace,
+                        size_t workspace_bytes, int32_t use_graph, void* stream);
+
+/* mean((a - b)^2) over n values, accumulated in FP64 into *out (device).
+ * The default stopping criterion of parsmooth/methods.py:50-51. */
+int psm_mean_sq_diff(int32_t dtype, int64_t n, const void* a, const void* b, double* out,
+                     void* stream);
+
+/* Drop every cached hipGraph (call before freeing buffers they reference). */
+void psm_clear_graphs(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* PSM_H_ */

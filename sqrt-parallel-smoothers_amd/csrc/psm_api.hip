@@ -1,0 +1,335 @@
+// extern "C" entry points of libparsmooth_hip.so (declared in include/psm.h):
+// argument checking, dispatch to the compiled (dtype, form, nx, ny) slice,
+// the per-thread error string, the stopping-criterion reduction and the
+// hipGraph cache of psm_filter_smoother.
+#include <hip/hip_runtime.h>
+
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "psm.h"
+#include "psm_impl.hpp"
+#include "psm_registry.hpp"
+
+#define PSM_DECL(T, NX, NY) const psm::OpsTable* psm_ops_##T##_##NX##_##NY();
+PSM_FOR_ALL(PSM_DECL)
+#undef PSM_DECL
+
+namespace psm {
+
+static thread_local std::string g_err;
+void set_error(const std::string& msg) { g_err = msg; }
+
+static const OpsTable* lookup(int dtype, int form, int nx, int ny) {
+  if (nx < 1 || nx > PSM_MAX_NX || ny < 1 || ny > PSM_MAX_NY) return nullptr;
+  if (dtype != PSM_F32 && dtype != PSM_F64) return nullptr;
+  if (form != PSM_FORM_SQRT && form != PSM_FORM_STD) return nullptr;
+  static const OpsTable* table[2][2][PSM_MAX_NX + 1][PSM_MAX_NY + 1] = {};
+  static std::once_flag once;
+  std::call_once(once, [] {
+#define PSM_FILL(T, NX, NY) fill_##T(NX, NY, psm_ops_##T##_##NX##_##NY());
+    auto fill_f32s = [](int nx, int ny, const OpsTable* t) { table[0][0][nx][ny] = t; };
+    auto fill_f32c = [](int nx, int ny, const OpsTable* t) { table[0][1][nx][ny] = t; };
+    auto fill_f64s = [](int nx, int ny, const OpsTable* t) { table[1][0][nx][ny] = t; };
+    auto fill_f64c = [](int nx, int ny, const OpsTable* t) { table[1][1][nx][ny] = t; };
+    PSM_FOR_ALL(PSM_FILL)
+#undef PSM_FILL
+  });
+  return table[dtype][form][nx][ny];
+}
+
+static const OpsTable* need(int dtype, int form, int nx, int ny) {
+  const OpsTable* t = lookup(dtype, form, nx, ny);
+  if (!t)
+    set_error("unsupported combination dtype=" + std::to_string(dtype) + " form=" +
+              std::to_string(form) + " nx=" + std::to_string(nx) + " ny=" + std::to_string(ny));
+  return t;
+}
+
+static int check_model(const psm_model_t* m, int nin, int nout) {
+  if (!m) { set_error("null model"); return PSM_E_ARG; }
+  if (m->nin != nin || m->nout != nout) {
+    set_error("model dims (" + std::to_string(m->nin) + "->" + std::to_string(m->nout) +
+              ") do not match (" + std::to_string(nin) + "->" + std::to_string(nout) + ")");
+    return PSM_E_ARG;
+  }
+  if (!m->q_mean || !m->q_chol) { set_error("model noise pointers are null"); return PSM_E_ARG; }
+  switch (m->kind) {
+    case PSM_MODEL_LINEAR:
+      if (!m->params) { set_error("linear model needs its matrix"); return PSM_E_ARG; }
+      return PSM_OK;
+    case PSM_MODEL_COORDINATED_TURN:
+      if (nin != 5 || nout != 5) { set_error("coordinated turn needs nx = 5"); return PSM_E_ARG; }
+      if (!m->params) { set_error("coordinated turn needs dt"); return PSM_E_ARG; }
+      return PSM_OK;
+    case PSM_MODEL_BEARINGS:
+      if (nout != 2 || nin < 2) { set_error("bearings needs ny = 2 and nx >= 2"); return PSM_E_ARG; }
+      if (!m->params) { set_error("bearings needs sensor positions"); return PSM_E_ARG; }
+      return PSM_OK;
+    case PSM_MODEL_SIN: case PSM_MODEL_COS: case PSM_MODEL_EXP: case PSM_MODEL_ARCTAN:
+      if (nin != nout) { set_error("elementwise model needs nin == nout"); return PSM_E_ARG; }
+      return PSM_OK;
+    default:
+      set_error("unknown model kind " + std::to_string(m->kind));
+      return PSM_E_ARG;
+  }
+}
+
+static int check_problem(const psm_problem_t* p) {
+  if (!p) { set_error("null problem"); return PSM_E_ARG; }
+  if (p->n_seq < 1 || p->n_states < p->n_seq || !p->state_offsets) {
+    set_error("bad sequence layout");
+    return PSM_E_ARG;
+  }
+  if (p->chunk < 0) { set_error("chunk must be >= 0"); return PSM_E_ARG; }
+  return PSM_OK;
+}
+
+// ---------------------------------------------------------------- criterion
+template <typename R>
+__global__ __launch_bounds__(1024) void k_mean_sq_diff(int64_t n, const R* __restrict__ a,
+                                                       const R* __restrict__ b,
+                                                       double* __restrict__ out) {
+  __shared__ double red[1024];
+  double acc = 0.0;
+  for (int64_t i = threadIdx.x; i < n; i += blockDim.x) {
+    const double d = (double)a[i] - (double)b[i];
+    acc += d * d;
+  }
+  red[threadIdx.x] = acc;
+  __syncthreads();
+  for (int w = 512; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *out = n > 0 ? red[0] / (double)n : 0.0;
+}
+
+// ---------------------------------------------------------------- graph cache
+struct GraphEntry {
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+};
+static std::mutex g_graph_mu;
+static std::map<std::vector<uint64_t>, GraphEntry> g_graphs;
+static hipStream_t g_side_stream = nullptr;
+
+static void push(std::vector<uint64_t>& k, const void* p) { k.push_back((uint64_t)(uintptr_t)p); }
+static void push_model(std::vector<uint64_t>& k, const psm_model_t* m) {
+  if (!m) { k.push_back(~0ull); return; }
+  k.push_back((uint64_t)m->kind);
+  k.push_back(((uint64_t)m->nin << 32) | (uint32_t)m->nout);
+  push(k, m->params); push(k, m->q_mean); push(k, m->q_chol);
+}
+
+}  // namespace psm
+
+using namespace psm;
+
+extern "C" {
+
+int psm_version(void) { return 1; }
+
+const char* psm_last_error(void) { return g_err.c_str(); }
+
+int psm_supported(int32_t dtype, int32_t form, int32_t nx, int32_t ny) {
+  return lookup(dtype, form, nx, ny) != nullptr ? 1 : 0;
+}
+
+int psm_linearize(int32_t dtype, int32_t form, const psm_model_t* model, const psm_sigma_t* sigma,
+                  int64_t n, const void* x_mean, const void* x_chol, void* F, void* C, void* b,
+                  void* stream) {
+  if (!model) { set_error("null model"); return PSM_E_ARG; }
+  const int nx = model->nin, nout = model->nout;
+  int rc = check_model(model, nx, nout);
+  if (rc) return rc;
+  if (nout == nx) {
+    const OpsTable* t = need(dtype, form, nx, 1);
+    if (!t) return PSM_E_UNSUPPORTED;
+    return t->linearize_trans(model, sigma, n, x_mean, x_chol, F, C, b, (hipStream_t)stream);
+  }
+  const OpsTable* t = need(dtype, form, nx, nout);
+  if (!t) return PSM_E_UNSUPPORTED;
+  return t->linearize_obs(model, sigma, n, x_mean, x_chol, F, C, b, (hipStream_t)stream);
+}
+
+int psm_filter_elements(int32_t dtype, int32_t form, int32_t nx, int32_t ny, int64_t n,
+                        const void* F, const void* cq, const void* bt, const void* H,
+                        const void* cr, const void* c, const void* m, const void* L,
+                        const void* y, void* A, void* b, void* U, void* eta, void* Z,
+                        void* stream) {
+  const OpsTable* t = need(dtype, form, nx, ny);
+  if (!t) return PSM_E_UNSUPPORTED;
+  return t->filter_elements(n, F, cq, bt, H, cr, c, m, L, y, A, b, U, eta, Z,
+                            (hipStream_t)stream);
+}
+
+int psm_filter_combine(int32_t dtype, int32_t form, int32_t nx, int64_t n, const void* const* e1,
+                       const void* const* e2, void* const* out, void* stream) {
+  const OpsTable* t = need(dtype, form, nx, 1);
+  if (!t) return PSM_E_UNSUPPORTED;
+  if (!e1 || !e2 || !out) { set_error("null element pointer array"); return PSM_E_ARG; }
+  return t->filter_combine(n, e1, e2, out, (hipStream_t)stream);
+}
+
+int psm_smoother_elements(int32_t dtype, int32_t form, int32_t nx, int64_t n, const void* F,
+                          const void* cq, const void* bt, const void* m, const void* L, void* g,
+                          void* E, void* D, void* stream) {
+  const OpsTable* t = need(dtype, form, nx, 1);
+  if (!t) return PSM_E_UNSUPPORTED;
+  return t->smoother_elements(n, F, cq, bt, m, L, g, E, D, (hipStream_t)stream);
+}
+
+int psm_smoother_combine(int32_t dtype, int32_t form, int32_t nx, int64_t n, const void* const* e1,
+                         const void* const* e2, void* const* out, void* stream) {
+  const OpsTable* t = need(dtype, form, nx, 1);
+  if (!t) return PSM_E_UNSUPPORTED;
+  if (!e1 || !e2 || !out) { set_error("null element pointer array"); return PSM_E_ARG; }
+  return t->smoother_combine(n, e1, e2, out, (hipStream_t)stream);
+}
+
+size_t psm_workspace_bytes(const psm_problem_t* p) {
+  if (check_problem(p)) return 0;
+  const OpsTable* t = need(p->dtype, p->form, p->nx, p->ny);
+  if (!t) return 0;
+  return t->workspace_bytes(p);
+}
+
+int psm_filter(const psm_problem_t* p, const psm_model_t* trans, const psm_model_t* obs,
+               const psm_sigma_t* sigma, const void* ys, const void* m0, const void* L0,
+               const void* nom_m, const void* nom_L, void* out_m, void* out_L, double* ell,
+               void* ws, size_t wsb, void* stream) {
+  int rc = check_problem(p);
+  if (rc) return rc;
+  if ((rc = check_model(trans, p->nx, p->nx))) return rc;
+  if ((rc = check_model(obs, p->nx, p->ny))) return rc;
+  const OpsTable* t = need(p->dtype, p->form, p->nx, p->ny);
+  if (!t) return PSM_E_UNSUPPORTED;
+  return t->filter(p, trans, obs, sigma, ys, m0, L0, nom_m, nom_L, out_m, out_L, ell, ws, wsb,
+                   (hipStream_t)stream, 0);
+}
+
+int psm_smoother(const psm_problem_t* p, const psm_model_t* trans, const psm_sigma_t* sigma,
+                 const void* fm, const void* fL, const void* nom_m, const void* nom_L, void* out_m,
+                 void* out_L, void* ws, size_t wsb, void* stream) {
+  int rc = check_problem(p);
+  if (rc) return rc;
+  if ((rc = check_model(trans, p->nx, p->nx))) return rc;
+  const OpsTable* t = need(p->dtype, p->form, p->nx, p->ny);
+  if (!t) return PSM_E_UNSUPPORTED;
+  return t->smoother(p, trans, sigma, fm, fL, nom_m, nom_L, out_m, out_L, ws, wsb,
+                     (hipStream_t)stream, 0);
+}
+
+int psm_filter_smoother(const psm_problem_t* p, const psm_model_t* trans, const psm_model_t* obs,
+                        const psm_sigma_t* sigma, const void* ys, const void* m0, const void* L0,
+                        const void* nom_m, const void* nom_L, void* filt_m, void* filt_L,
+                        void* out_m, void* out_L, void* ws, size_t wsb, int32_t use_graph,
+                        void* stream) {
+  int rc = check_problem(p);
+  if (rc) return rc;
+  if ((rc = check_model(trans, p->nx, p->nx))) return rc;
+  if ((rc = check_model(obs, p->nx, p->ny))) return rc;
+  const OpsTable* t = need(p->dtype, p->form, p->nx, p->ny);
+  if (!t) return PSM_E_UNSUPPORTED;
+  auto run = [&](hipStream_t s) -> int {
+    int r = t->filter(p, trans, obs, sigma, ys, m0, L0, nom_m, nom_L, filt_m, filt_L, nullptr, ws,
+                      wsb, s, 0);
+    if (r) return r;
+    return t->smoother(p, trans, sigma, filt_m, filt_L, nom_m, nom_L, out_m, out_L, ws, wsb, s, 1);
+  };
+  hipStream_t st = (hipStream_t)stream;
+  if (!use_graph) return run(st);
+
+  std::vector<uint64_t> key;
+  key.push_back(((uint64_t)p->dtype << 48) | ((uint64_t)p->form << 40) | ((uint64_t)p->nx << 32) |
+                ((uint64_t)p->ny << 16) | (uint64_t)(uint16_t)p->chunk);
+  key.push_back((uint64_t)p->n_seq);
+  key.push_back((uint64_t)p->n_states);
+  push(key, p->state_offsets);
+  push_model(key, trans);
+  push_model(key, obs);
+  if (sigma && sigma->npts > 0) {
+    key.push_back((uint64_t)sigma->npts);
+    push(key, sigma->xi); push(key, sigma->wm); push(key, sigma->wc);
+  } else {
+    key.push_back(0);
+  }
+  for (const void* q : {ys, m0, L0, nom_m, nom_L, (const void*)filt_m, (const void*)filt_L,
+                        (const void*)out_m, (const void*)out_L, (const void*)ws})
+    push(key, q);
+  key.push_back((uint64_t)wsb);
+  push(key, stream);
+
+  std::lock_guard<std::mutex> lock(g_graph_mu);
+  // Capture and replay on a non-default stream; order it after/before the
+  // caller's stream with events so the legacy default stream works too.
+  if (g_side_stream == nullptr) {
+    if (hipStreamCreateWithFlags(&g_side_stream, hipStreamNonBlocking) != hipSuccess) {
+      set_error("hipStreamCreate failed");
+      return PSM_E_HIP;
+    }
+  }
+  hipStream_t cs = g_side_stream;
+  auto it = g_graphs.find(key);
+  if (it == g_graphs.end()) {
+    GraphEntry ge;
+    if (hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+      set_error("hipStreamBeginCapture failed");
+      return PSM_E_HIP;
+    }
+    int r = run(cs);
+    hipError_t e = hipStreamEndCapture(cs, &ge.graph);
+    if (r) { if (ge.graph) (void)hipGraphDestroy(ge.graph); return r; }
+    if (e != hipSuccess) { set_error(std::string("hipStreamEndCapture: ") + hipGetErrorString(e)); return PSM_E_HIP; }
+    e = hipGraphInstantiate(&ge.exec, ge.graph, nullptr, nullptr, 0);
+    if (e != hipSuccess) {
+      (void)hipGraphDestroy(ge.graph);
+      set_error(std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+      return PSM_E_HIP;
+    }
+    it = g_graphs.emplace(key, ge).first;
+  }
+  hipEvent_t ev_in, ev_out;
+  (void)hipEventCreateWithFlags(&ev_in, hipEventDisableTiming);
+  (void)hipEventCreateWithFlags(&ev_out, hipEventDisableTiming);
+  (void)hipEventRecord(ev_in, st);
+  (void)hipStreamWaitEvent(cs, ev_in, 0);
+  hipError_t e = hipGraphLaunch(it->second.exec, cs);
+  (void)hipEventRecord(ev_out, cs);
+  (void)hipStreamWaitEvent(st, ev_out, 0);
+  (void)hipEventDestroy(ev_in);
+  (void)hipEventDestroy(ev_out);
+  if (e != hipSuccess) {
+    set_error(std::string("hipGraphLaunch: ") + hipGetErrorString(e));
+    return PSM_E_HIP;
+  }
+  return PSM_OK;
+}
+
+int psm_mean_sq_diff(int32_t dtype, int64_t n, const void* a, const void* b, double* out,
+                     void* stream) {
+  hipStream_t st = (hipStream_t)stream;
+  if (dtype == PSM_F64)
+    hipLaunchKernelGGL(k_mean_sq_diff<double>, dim3(1), dim3(1024), 0, st, n, (const double*)a,
+                       (const double*)b, out);
+  else if (dtype == PSM_F32)
+    hipLaunchKernelGGL(k_mean_sq_diff<float>, dim3(1), dim3(1024), 0, st, n, (const float*)a,
+                       (const float*)b, out);
+  else { set_error("bad dtype"); return PSM_E_ARG; }
+  return hip_check("k_mean_sq_diff");
+}
+
+void psm_clear_graphs(void) {
+  std::lock_guard<std::mutex> lock(g_graph_mu);
+  for (auto& kv : g_graphs) {
+    if (kv.second.exec) (void)hipGraphExecDestroy(kv.second.exec);
+    if (kv.second.graph) (void)hipGraphDestroy(kv.second.graph);
+  }
+  g_graphs.clear();
+}
+
+}  // extern "C"

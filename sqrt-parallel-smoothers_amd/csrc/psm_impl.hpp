@@ -1,0 +1,345 @@
+// Host-side launchers for one (real type, nx, ny, form) instantiation, and the
+// function table the C ABI dispatches through.
+#pragma once
+#include <cstdio>
+#include <cstdlib>
+#include <string>
+
+#include "psm_kernels.hpp"
+
+namespace psm {
+
+// ---------------------------------------------------------------- errors
+void set_error(const std::string& msg);   // defined in psm_api.cpp
+
+inline int hip_check(const char* what) {
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) {
+    set_error(std::string(what) + ": " + hipGetErrorString(e));
+    return PSM_E_HIP;
+  }
+  return PSM_OK;
+}
+
+// ---------------------------------------------------------------- workspace plan
+struct ScanLevel {
+  int64_t n;      // items at this level
+  int K;          // items per lane
+  int64_t nwg;    // workgroups
+  size_t off_in;  // byte offsets in the workspace
+  size_t off_out;
+};
+
+struct Plan {
+  int64_t M = 0;
+  int chunk = 0;
+  int64_t nchunks = 0;
+  int nlevels = 0;
+  ScanLevel lv[16];
+  size_t off_lin = 0, off_ell = 0, off_selem = 0, total = 0;
+};
+
+inline int env_int(const char* name, int dflt) {
+  const char* v = std::getenv(name);
+  return v ? std::atoi(v) : dflt;
+}
+
+inline size_t align_up(size_t x) { return (x + 255) & ~size_t(255); }
+
+// Default chunk (states per lane in the reduce / apply passes): small enough to
+// keep every CU busy on a single long sequence, larger when the batch is big.
+inline int default_chunk(int64_t M) {
+  if (M >= (int64_t)1 << 22) return 16;
+  if (M >= (int64_t)1 << 18) return 8;
+  return 4;
+}
+
+inline Plan make_plan(const psm_problem_t* p, size_t rsize, int nlin, int nqf, int nqs) {
+  Plan pl;
+  pl.M = p->n_states;
+  pl.chunk = p->chunk > 0 ? p->chunk : env_int("PSM_CHUNK", default_chunk(pl.M));
+  pl.nchunks = (pl.M + pl.chunk - 1) / pl.chunk;
+  const int nq = nqf > nqs ? nqf : nqs;
+  size_t off = 0;
+  pl.off_lin = off;
+  off = align_up(off + (size_t)nlin * pl.M * rsize);
+  pl.off_ell = off;
+  off = align_up(off + (size_t)pl.M * sizeof(double));
+  pl.off_selem = off;
+  off = align_up(off + (size_t)nqs * pl.M * rsize);
+  const int kmax = env_int("PSM_SCAN_K", 4);
+  int64_t n = pl.nchunks;
+  int l = 0;
+  while (true) {
+    ScanLevel& s = pl.lv[l];
+    s.n = n;
+    if (n <= (int64_t)kScanWG * 16) {
+      s.K = (int)((n + kScanWG - 1) / kScanWG);
+      if (s.K < 1) s.K = 1;
+      s.nwg = 1;
+    } else {
+      s.K = kmax;
+      s.nwg = (n + (int64_t)kScanWG * kmax - 1) / ((int64_t)kScanWG * kmax);
+    }
+    s.off_in = off;
+    off = align_up(off + (size_t)nq * n * rsize);
+    s.off_out = off;
+    off = align_up(off + (size_t)nq * n * rsize);
+    ++l;
+    if (s.nwg == 1 || l == 16) break;
+    n = s.nwg;
+  }
+  pl.nlevels = l;
+  pl.total = off;
+  return pl;
+}
+
+template <typename R>
+inline ModelView<R> model_view(const psm_model_t* m) {
+  ModelView<R> v;
+  v.kind = m->kind;
+  v.params = static_cast<const R*>(m->params);
+  v.q_mean = static_cast<const R*>(m->q_mean);
+  v.q_chol = static_cast<const R*>(m->q_chol);
+  return v;
+}
+
+template <typename R>
+inline SigmaView<R> sigma_view(const psm_sigma_t* s, int* use) {
+  SigmaView<R> v{0, nullptr, nullptr, nullptr};
+  *use = 0;
+  if (s != nullptr && s->npts > 0) {
+    v.npts = s->npts;
+    v.xi = static_cast<const R*>(s->xi);
+    v.wm = static_cast<const R*>(s->wm);
+    v.wc = static_cast<const R*>(s->wc);
+    *use = 1;
+  }
+  return v;
+}
+
+inline unsigned grid_for(int64_t n, int wg) { return (unsigned)((n + wg - 1) / wg); }
+
+// ---------------------------------------------------------------- the function table
+struct OpsTable {
+  int (*linearize_trans)(const psm_model_t*, const psm_sigma_t*, int64_t, const void*, const void*,
+                         void*, void*, void*, hipStream_t);
+  int (*linearize_obs)(const psm_model_t*, const psm_sigma_t*, int64_t, const void*, const void*,
+                       void*, void*, void*, hipStream_t);
+  int (*filter_elements)(int64_t, const void*, const void*, const void*, const void*, const void*,
+                         const void*, const void*, const void*, const void*, void*, void*, void*,
+                         void*, void*, hipStream_t);
+  int (*filter_combine)(int64_t, const void* const*, const void* const*, void* const*, hipStream_t);
+  int (*smoother_elements)(int64_t, const void*, const void*, const void*, const void*, const void*,
+                           void*, void*, void*, hipStream_t);
+  int (*smoother_combine)(int64_t, const void* const*, const void* const*, void* const*,
+                          hipStream_t);
+  size_t (*workspace_bytes)(const psm_problem_t*);
+  int (*filter)(const psm_problem_t*, const psm_model_t*, const psm_model_t*, const psm_sigma_t*,
+                const void*, const void*, const void*, const void*, const void*, void*, void*,
+                double*, void*, size_t, hipStream_t, int lin_done);
+  int (*smoother)(const psm_problem_t*, const psm_model_t*, const psm_sigma_t*, const void*,
+                  const void*, const void*, const void*, void*, void*, void*, size_t, hipStream_t,
+                  int lin_done);
+};
+
+template <typename R, int NX, int NY, bool SQRT>
+struct Impl {
+  using FA = FAgg<R, NX>;
+  using SA = SAgg<R, NX>;
+  static constexpr int NLIN = Step<R, NX, NY>::NQ;
+
+  static Plan plan(const psm_problem_t* p) {
+    return make_plan(p, sizeof(R), NLIN, FA::NQ, SA::NQ);
+  }
+
+  static size_t workspace_bytes(const psm_problem_t* p) { return plan(p).total; }
+
+  template <int NIN, int NOUT>
+  static int linearize_points(const psm_model_t* md, const psm_sigma_t* sg, int64_t n,
+                              const void* xm, const void* xL, void* F, void* C, void* b,
+                              hipStream_t st) {
+    if (n <= 0) return PSM_OK;
+    int use = 0;
+    SigmaView<R> sv = sigma_view<R>(sg, &use);
+    hipLaunchKernelGGL((k_linearize_points<R, NIN, NOUT, SQRT>), dim3(grid_for(n, kLaneWG)),
+                       dim3(kLaneWG), 0, st, n, model_view<R>(md), sv, use,
+                       static_cast<const R*>(xm), static_cast<const R*>(xL), static_cast<R*>(F),
+                       static_cast<R*>(C), static_cast<R*>(b));
+    return hip_check("k_linearize_points");
+  }
+  static int linearize_trans(const psm_model_t* md, const psm_sigma_t* sg, int64_t n,
+                             const void* xm, const void* xL, void* F, void* C, void* b,
+                             hipStream_t st) {
+    return linearize_points<NX, NX>(md, sg, n, xm, xL, F, C, b, st);
+  }
+  static int linearize_obs(const psm_model_t* md, const psm_sigma_t* sg, int64_t n, const void* xm,
+                           const void* xL, void* F, void* C, void* b, hipStream_t st) {
+    return linearize_points<NX, NY>(md, sg, n, xm, xL, F, C, b, st);
+  }
+
+  static int filter_elements(int64_t n, const void* F, const void* cq, const void* bt,
+                             const void* H, const void* cr, const void* c, const void* m,
+                             const void* L, const void* y, void* A, void* b, void* U, void* eta,
+                             void* Z, hipStream_t st) {
+    if (n <= 0) return PSM_OK;
+    hipLaunchKernelGGL((k_filter_elements<R, NX, NY, SQRT>), dim3(grid_for(n, kLaneWG)),
+                       dim3(kLaneWG), 0, st, n, (const R*)F, (const R*)cq, (const R*)bt,
+                       (const R*)H, (const R*)cr, (const R*)c, (const R*)m, (const R*)L,
+                       (const R*)y, (R*)A, (R*)b, (R*)U, (R*)eta, (R*)Z);
+    return hip_check("k_filter_elements");
+  }
+
+  static int filter_combine(int64_t n, const void* const* e1, const void* const* e2,
+                            void* const* out, hipStream_t st) {
+    if (n <= 0) return PSM_OK;
+    FPtrs<R, NX> a, b;
+    FOut<R, NX> o;
+    for (int k = 0; k < 5; ++k) {
+      a.p[k] = (const R*)e1[k];
+      b.p[k] = (const R*)e2[k];
+      o.p[k] = (R*)out[k];
+    }
+    hipLaunchKernelGGL((k_filter_combine<R, NX, SQRT>), dim3(grid_for(n, kLaneWG)), dim3(kLaneWG),
+                       0, st, n, a, b, o);
+    return hip_check("k_filter_combine");
+  }
+
+  static int smoother_elements(int64_t n, const void* F, const void* cq, const void* bt,
+                               const void* m, const void* L, void* g, void* E, void* D,
+                               hipStream_t st) {
+    if (n <= 0) return PSM_OK;
+    hipLaunchKernelGGL((k_smoother_elements<R, NX, SQRT>), dim3(grid_for(n, kLaneWG)),
+                       dim3(kLaneWG), 0, st, n, (const R*)F, (const R*)cq, (const R*)bt,
+                       (const R*)m, (const R*)L, (R*)g, (R*)E, (R*)D);
+    return hip_check("k_smoother_elements");
+  }
+
+  static int smoother_combine(int64_t n, const void* const* e1, const void* const* e2,
+                              void* const* out, hipStream_t st) {
+    if (n <= 0) return PSM_OK;
+    SPtrs<R, NX> a, b;
+    SOut<R, NX> o;
+    for (int k = 0; k < 3; ++k) {
+      a.p[k] = (const R*)e1[k];
+      b.p[k] = (const R*)e2[k];
+      o.p[k] = (R*)out[k];
+    }
+    hipLaunchKernelGGL((k_smoother_combine<R, NX, SQRT>), dim3(grid_for(n, kLaneWG)),
+                       dim3(kLaneWG), 0, st, n, a, b, o);
+    return hip_check("k_smoother_combine");
+  }
+
+  template <typename Agg>
+  static int run_scan(const Plan& pl, char* ws, int reverse, hipStream_t st) {
+    for (int l = 0; l < pl.nlevels; ++l) {
+      const ScanLevel& s = pl.lv[l];
+      R* in = (R*)(ws + s.off_in);
+      R* out = (R*)(ws + s.off_out);
+      R* tot = (l + 1 < pl.nlevels) ? (R*)(ws + pl.lv[l + 1].off_in) : nullptr;
+      hipLaunchKernelGGL((k_scan_up<Agg, R, SQRT>), dim3((unsigned)s.nwg), dim3(kScanWG), 0, st,
+                         (const R*)in, s.n, s.K, (l == 0) ? reverse : 0, out, tot, s.nwg);
+      int rc = hip_check("k_scan_up");
+      if (rc) return rc;
+    }
+    for (int l = pl.nlevels - 2; l >= 0; --l) {
+      const ScanLevel& s = pl.lv[l];
+      hipLaunchKernelGGL((k_scan_down<Agg, R, SQRT>), dim3((unsigned)s.nwg), dim3(kScanWG), 0, st,
+                         (R*)(ws + s.off_out), s.n, s.K, (l == 0) ? reverse : 0,
+                         (const R*)(ws + pl.lv[l + 1].off_out), pl.lv[l + 1].n);
+      int rc = hip_check("k_scan_down");
+      if (rc) return rc;
+    }
+    return PSM_OK;
+  }
+
+  static int linearize_states(const Plan& pl, const psm_model_t* tr, const psm_model_t* ob,
+                              const psm_sigma_t* sg, const void* nm, const void* nL, char* ws,
+                              hipStream_t st) {
+    int use = 0;
+    SigmaView<R> sv = sigma_view<R>(sg, &use);
+    ModelView<R> obv = ob ? model_view<R>(ob) : model_view<R>(tr);
+    hipLaunchKernelGGL((k_linearize_states<R, NX, NY, SQRT>), dim3(grid_for(pl.M, kLaneWG)),
+                       dim3(kLaneWG), 0, st, pl.M, model_view<R>(tr), obv, ob ? 1 : 0, sv, use,
+                       (const R*)nm, (const R*)nL, (R*)(ws + pl.off_lin));
+    return hip_check("k_linearize_states");
+  }
+
+  static int filter(const psm_problem_t* p, const psm_model_t* tr, const psm_model_t* ob,
+                    const psm_sigma_t* sg, const void* ys, const void* m0, const void* L0,
+                    const void* nm, const void* nL, void* om, void* oL, double* ell, void* wsv,
+                    size_t wsb, hipStream_t st, int lin_done) {
+    const Plan pl = plan(p);
+    if (wsb < pl.total) {
+      set_error("workspace too small");
+      return PSM_E_WORKSPACE;
+    }
+    if (pl.M <= 0) return PSM_OK;
+    char* ws = (char*)wsv;
+    int rc;
+    if (!lin_done) {
+      rc = linearize_states(pl, tr, ob, sg, nm, nL, ws, st);
+      if (rc) return rc;
+    }
+    const R* lin = (const R*)(ws + pl.off_lin);
+    hipLaunchKernelGGL((k_filter_reduce<R, NX, NY, SQRT>), dim3(grid_for(pl.nchunks, kLaneWG)),
+                       dim3(kLaneWG), 0, st, pl.M, pl.chunk, pl.nchunks, p->state_offsets,
+                       p->n_seq, lin, (const R*)ys, (const R*)m0, (const R*)L0,
+                       (R*)(ws + pl.lv[0].off_in));
+    if ((rc = hip_check("k_filter_reduce"))) return rc;
+    if ((rc = run_scan<FA>(pl, ws, 0, st))) return rc;
+    double* ell_state = ell ? (double*)(ws + pl.off_ell) : nullptr;
+    hipLaunchKernelGGL((k_filter_apply<R, NX, NY, SQRT>), dim3(grid_for(pl.nchunks, kLaneWG)),
+                       dim3(kLaneWG), 0, st, pl.M, pl.chunk, pl.nchunks, p->state_offsets,
+                       p->n_seq, lin, (const R*)ys, (const R*)m0, (const R*)L0,
+                       (const R*)(ws + pl.lv[0].off_out), (R*)om, (R*)oL, ell_state);
+    if ((rc = hip_check("k_filter_apply"))) return rc;
+    if (ell) {
+      hipLaunchKernelGGL(k_ell_reduce<256>, dim3((unsigned)p->n_seq), dim3(256), 0, st,
+                         (const double*)ell_state, p->state_offsets, ell);
+      if ((rc = hip_check("k_ell_reduce"))) return rc;
+    }
+    return PSM_OK;
+  }
+
+  static int smoother(const psm_problem_t* p, const psm_model_t* tr, const psm_sigma_t* sg,
+                      const void* fm, const void* fL, const void* nm, const void* nL, void* om,
+                      void* oL, void* wsv, size_t wsb, hipStream_t st, int lin_done) {
+    const Plan pl = plan(p);
+    if (wsb < pl.total) {
+      set_error("workspace too small");
+      return PSM_E_WORKSPACE;
+    }
+    if (pl.M <= 0) return PSM_OK;
+    char* ws = (char*)wsv;
+    int rc;
+    if (!lin_done) {
+      rc = linearize_states(pl, tr, nullptr, sg, nm, nL, ws, st);
+      if (rc) return rc;
+    }
+    hipLaunchKernelGGL((k_smoother_reduce<R, NX, NY, SQRT>), dim3(grid_for(pl.nchunks, kLaneWG)),
+                       dim3(kLaneWG), 0, st, pl.M, pl.chunk, pl.nchunks, p->state_offsets,
+                       p->n_seq, (const R*)(ws + pl.off_lin), (const R*)fm, (const R*)fL,
+                       (R*)(ws + pl.off_selem), (R*)(ws + pl.lv[0].off_in));
+    if ((rc = hip_check("k_smoother_reduce"))) return rc;
+    if ((rc = run_scan<SA>(pl, ws, 1, st))) return rc;
+    hipLaunchKernelGGL((k_smoother_apply<R, NX, SQRT>), dim3(grid_for(pl.nchunks, kLaneWG)),
+                       dim3(kLaneWG), 0, st, pl.M, pl.chunk, pl.nchunks,
+                       (const R*)(ws + pl.off_selem), (const R*)(ws + pl.lv[0].off_out), (R*)om,
+                       (R*)oL);
+    return hip_check("k_smoother_apply");
+  }
+
+  static const OpsTable* table() {
+    static const OpsTable t = {&linearize_trans, &linearize_obs, &filter_elements,
+                               &filter_combine,  &smoother_elements, &smoother_combine,
+                               &workspace_bytes, &filter,          &smoother};
+    return &t;
+  }
+};
+
+}  // namespace psm
+
+// One translation unit instantiates every ny for a given (dtype, form, nx).
+#define PSM_INSTANTIATE(TAG, R, NX, NY, SQRT)                        \
+  const psm::OpsTable* TAG() { return psm::Impl<R, NX, NY, SQRT>::table(); }

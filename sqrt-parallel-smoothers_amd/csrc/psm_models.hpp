@@ -1,0 +1,308 @@
+// Built-in state-space model functions and the per-point linearizations
+// (extended / sigma-point statistical linear regression) evaluated on device.
+//
+// A Python callable cannot run inside a kernel, so the library carries a small
+// registry of the functions the reference's own tests, notebooks and
+// benchmarks use (tests/_lgssm.py:5-39, tests/bearings/bearings_utils.py:7-69,
+// tests/test_linearization.py:157), each with an analytic Jacobian.
+#pragma once
+#include "psm_linalg.hpp"
+#include "psm.h"
+
+namespace psm {
+
+// Device view of a psm_model_t (see include/psm.h).
+template <typename R>
+struct ModelView {
+  int kind;
+  const R* params;   // model parameters (device)
+  const R* q_mean;   // additive noise mean (device, nout)
+  const R* q_chol;   // noise chol (sqrt form) or covariance (covariance form), nout x nout
+};
+
+template <typename R>
+PSM_DEV R atan2_(R y, R x);
+template <>
+PSM_DEV double atan2_<double>(double y, double x) { return atan2(y, x); }
+template <>
+PSM_DEV float atan2_<float>(float y, float x) { return atan2f(y, x); }
+PSM_DEV double sin_(double x) { return sin(x); }
+PSM_DEV float sin_(float x) { return sinf(x); }
+PSM_DEV double cos_(double x) { return cos(x); }
+PSM_DEV float cos_(float x) { return cosf(x); }
+PSM_DEV double exp_(double x) { return exp(x); }
+PSM_DEV float exp_(float x) { return expf(x); }
+PSM_DEV double atan_(double x) { return atan(x); }
+PSM_DEV float atan_(float x) { return atanf(x); }
+
+// f(x) and, when JAC, its Jacobian J (NOUT x NIN).
+template <typename R, int NIN, int NOUT, bool JAC>
+PSM_DEV void model_eval(const ModelView<R>& md, const R (&x)[NIN], R (&y)[NOUT], R (&J)[NOUT][NIN]) {
+  switch (md.kind) {
+    case PSM_MODEL_LINEAR: {
+      // x -> A x, A row-major NOUT x NIN
+#pragma unroll
+      for (int i = 0; i < NOUT; ++i) {
+        R s = R(0);
+#pragma unroll
+        for (int j = 0; j < NIN; ++j) {
+          const R a = md.params[i * NIN + j];
+          s = fma(a, x[j], s);
+          if (JAC) J[i][j] = a;
+        }
+        y[i] = s;
+      }
+      break;
+    }
+    case PSM_MODEL_COORDINATED_TURN: {
+      if constexpr (NIN == 5 && NOUT == 5) {
+        // tests/bearings/bearings_utils.py:7-44: branch at |w| < 1e-6 where
+        // sin(wt)/w -> dt and (cos(wt)-1)/w -> 0 with zero w-derivative.
+        const R dt = md.params[0];
+        const R w = x[4];
+        const R c = cos_(w * dt);
+        const R s = sin_(w * dt);
+        const bool small = abs_(w) < R(1e-6);
+        const R a = small ? dt : s / w;
+        const R bb = small ? R(0) : (c - R(1)) / w;
+        y[0] = x[0] + a * x[2] - bb * x[3];
+        y[1] = x[1] + bb * x[2] + a * x[3];
+        y[2] = c * x[2] + s * x[3];
+        y[3] = -s * x[2] + c * x[3];
+        y[4] = x[4];
+        if (JAC) {
+          const R da = small ? R(0) : (dt * c - a) / w;
+          const R db = small ? R(0) : (-dt * s - bb) / w;
+          const R dc = -dt * s, ds = dt * c;
+#pragma unroll
+          for (int i = 0; i < 5; ++i)
+#pragma unroll
+            for (int j = 0; j < 5; ++j) J[i][j] = R(0);
+          J[0][0] = R(1); J[0][2] = a;  J[0][3] = -bb; J[0][4] = da * x[2] - db * x[3];
+          J[1][1] = R(1); J[1][2] = bb; J[1][3] = a;   J[1][4] = db * x[2] + da * x[3];
+          J[2][2] = c;    J[2][3] = s;  J[2][4] = dc * x[2] + ds * x[3];
+          J[3][2] = -s;   J[3][3] = c;  J[3][4] = -ds * x[2] + dc * x[3];
+          J[4][4] = R(1);
+        }
+      }
+      break;
+    }
+    case PSM_MODEL_BEARINGS: {
+      if constexpr (NOUT == 2 && NIN >= 2) {
+        // tests/bearings/bearings_utils.py:47-69 (no angle wrapping)
+#pragma unroll
+        for (int r = 0; r < 2; ++r) {
+          const R dx = x[0] - md.params[2 * r];
+          const R dy = x[1] - md.params[2 * r + 1];
+          y[r] = atan2_(dy, dx);
+          if (JAC) {
+            const R q = dx * dx + dy * dy;
+#pragma unroll
+            for (int j = 0; j < NIN; ++j) J[r][j] = R(0);
+            J[r][0] = -dy / q;
+            J[r][1] = dx / q;
+          }
+        }
+      }
+      break;
+    }
+    default: {
+      // elementwise sin / cos / exp / arctan (NIN == NOUT)
+      if constexpr (NIN == NOUT) {
+#pragma unroll
+        for (int i = 0; i < NOUT; ++i) {
+          R v, d;
+          const R xi = x[i];
+          if (md.kind == PSM_MODEL_SIN) { v = sin_(xi); d = cos_(xi); }
+          else if (md.kind == PSM_MODEL_COS) { v = cos_(xi); d = -sin_(xi); }
+          else if (md.kind == PSM_MODEL_EXP) { v = exp_(xi); d = v; }
+          else { v = atan_(xi); d = R(1) / (R(1) + xi * xi); }
+          y[i] = v;
+          if (JAC) {
+#pragma unroll
+            for (int j = 0; j < NIN; ++j) J[i][j] = (i == j) ? d : R(0);
+          }
+        }
+      }
+      break;
+    }
+  }
+}
+
+template <typename R, int NIN, int NOUT>
+PSM_DEV void model_value(const ModelView<R>& md, const R (&x)[NIN], R (&y)[NOUT]) {
+  R J[NOUT][NIN];
+  model_eval<R, NIN, NOUT, false>(md, x, y, J);
+}
+
+// Sigma-point table (unit points xi_p, weights) in device memory.
+template <typename R>
+struct SigmaView {
+  int npts;
+  const R* xi;  // npts x n
+  const R* wm;
+  const R* wc;
+};
+
+// ----------------------------------------------------------------------------
+// Extended (first-order Taylor) linearization, parsmooth/linearization/_extended.py:59-70:
+// F = df/dx(m), chol/cov passed through, b = f(m) - F m + m_q.  The nominal
+// covariance is not used.
+// ----------------------------------------------------------------------------
+template <typename R, int NIN, int NOUT>
+PSM_DEV void linearize_extended(const ModelView<R>& md, const R (&m)[NIN], R (&F)[NOUT][NIN],
+                                R (&C)[NOUT][NOUT], R (&b)[NOUT]) {
+  R y[NOUT];
+  model_eval<R, NIN, NOUT, true>(md, m, y, F);
+#pragma unroll
+  for (int i = 0; i < NOUT; ++i) {
+    R s = y[i];
+#pragma unroll
+    for (int j = 0; j < NIN; ++j) s = fma(-F[i][j], m[j], s);
+    b[i] = s + md.q_mean[i];
+#pragma unroll
+    for (int j = 0; j < NOUT; ++j) C[i][j] = md.q_chol[i * NOUT + j];
+  }
+}
+
+// ----------------------------------------------------------------------------
+// Sigma-point statistical linear regression, parsmooth/linearization/_sigma_points.py:63-100.
+// Two passes over the points (the function is re-evaluated instead of storing
+// npts outputs): pass 1 forms the weighted mean, pass 2 the cross-covariance
+// and, in square-root form, streams sqrt(wc_p)(f_p - m_f) into a running
+// triangular factor that starts from the noise factor, which equals
+// tria([tria(sqrt_Phi) | chol_q]) of the reference.  The factor is then
+// downdated by the columns of F L (rank-one downdates, multiplier -1).
+// SQRT=false returns Phi - F P F^T + Q symmetrised (:80-85).
+// ----------------------------------------------------------------------------
+template <typename R, int NIN, int NOUT, bool SQRT>
+PSM_DEV void linearize_sigma(const ModelView<R>& md, const SigmaView<R>& sp, const R (&m)[NIN],
+                             const R (&L)[NIN][NIN], R (&F)[NOUT][NIN], R (&C)[NOUT][NOUT],
+                             R (&b)[NOUT]) {
+  R mf[NOUT];
+  set_zero(mf);
+  for (int p = 0; p < sp.npts; ++p) {
+    R chi[NIN];
+#pragma unroll
+    for (int i = 0; i < NIN; ++i) {
+      R s = m[i];
+#pragma unroll
+      for (int j = 0; j < NIN; ++j) s = fma(L[i][j], sp.xi[p * NIN + j], s);
+      chi[i] = s;
+    }
+    R f[NOUT];
+    model_value<R, NIN, NOUT>(md, chi, f);
+    const R w = sp.wm[p];
+#pragma unroll
+    for (int i = 0; i < NOUT; ++i) mf[i] = fma(w, f[i], mf[i]);
+  }
+  R Psi[NIN][NOUT];
+  set_zero(Psi);
+  R Acc[NOUT][NOUT + 1];   // running factor | new column
+  R Phi[NOUT][NOUT];
+  if constexpr (SQRT) {
+#pragma unroll
+    for (int i = 0; i < NOUT; ++i)
+#pragma unroll
+      for (int j = 0; j < NOUT; ++j) Acc[i][j] = md.q_chol[i * NOUT + j];
+    {
+      R sq[NOUT][NOUT];
+#pragma unroll
+      for (int i = 0; i < NOUT; ++i)
+#pragma unroll
+        for (int j = 0; j < NOUT; ++j) sq[i][j] = Acc[i][j];
+      tria_inplace<R, NOUT, NOUT>(sq);
+#pragma unroll
+      for (int i = 0; i < NOUT; ++i)
+#pragma unroll
+        for (int j = 0; j < NOUT; ++j) Acc[i][j] = sq[i][j];
+    }
+  } else {
+    set_zero(Phi);
+  }
+  for (int p = 0; p < sp.npts; ++p) {
+    R chi[NIN], dx[NIN];
+#pragma unroll
+    for (int i = 0; i < NIN; ++i) {
+      R s = m[i];
+#pragma unroll
+      for (int j = 0; j < NIN; ++j) s = fma(L[i][j], sp.xi[p * NIN + j], s);
+      chi[i] = s;
+      dx[i] = s - m[i];
+    }
+    R f[NOUT];
+    model_value<R, NIN, NOUT>(md, chi, f);
+    const R w = sp.wc[p];
+    R d[NOUT];
+#pragma unroll
+    for (int i = 0; i < NOUT; ++i) d[i] = f[i] - mf[i];
+#pragma unroll
+    for (int i = 0; i < NIN; ++i) {
+      const R wx = w * dx[i];
+#pragma unroll
+      for (int j = 0; j < NOUT; ++j) Psi[i][j] = fma(wx, d[j], Psi[i][j]);
+    }
+    if constexpr (SQRT) {
+      const R sw = sqrt_(w);
+#pragma unroll
+      for (int i = 0; i < NOUT; ++i) Acc[i][NOUT] = sw * d[i];
+      tria_inplace<R, NOUT, NOUT + 1>(Acc);
+    } else {
+#pragma unroll
+      for (int i = 0; i < NOUT; ++i)
+#pragma unroll
+        for (int j = 0; j < NOUT; ++j) Phi[i][j] = fma(w * d[i], d[j], Phi[i][j]);
+    }
+  }
+  // F = (L^-T L^-1 Psi)^T
+  R X[NIN][NOUT], Y[NIN][NOUT];
+  solve_lower<R, NIN, NOUT>(L, Psi, X);
+  solve_lower_T<R, NIN, NOUT>(L, X, Y);
+#pragma unroll
+  for (int i = 0; i < NOUT; ++i)
+#pragma unroll
+    for (int j = 0; j < NIN; ++j) F[i][j] = Y[j][i];
+#pragma unroll
+  for (int i = 0; i < NOUT; ++i) {
+    R s = mf[i];
+#pragma unroll
+    for (int j = 0; j < NIN; ++j) s = fma(-F[i][j], m[j], s);
+    b[i] = s + md.q_mean[i];
+  }
+  R FL[NOUT][NIN];
+  mm<R, NOUT, NIN, NIN>(F, L, FL);
+  if constexpr (SQRT) {
+#pragma unroll
+    for (int i = 0; i < NOUT; ++i)
+#pragma unroll
+      for (int j = 0; j < NOUT; ++j) C[i][j] = Acc[i][j];
+#pragma unroll
+    for (int k = 0; k < NIN; ++k) {
+      R v[NOUT];
+#pragma unroll
+      for (int i = 0; i < NOUT; ++i) v[i] = FL[i][k];
+      cholesky_update<R, NOUT>(C, v, R(-1));
+    }
+  } else {
+    // Phi - (F L)(F L)^T + Q, symmetrised
+#pragma unroll
+    for (int i = 0; i < NOUT; ++i)
+#pragma unroll
+      for (int j = 0; j < NOUT; ++j) {
+        R s = Phi[i][j] + md.q_chol[i * NOUT + j];
+#pragma unroll
+        for (int k = 0; k < NIN; ++k) s = fma(-FL[i][k], FL[j][k], s);
+        C[i][j] = s;
+      }
+#pragma unroll
+    for (int i = 0; i < NOUT; ++i)
+#pragma unroll
+      for (int j = 0; j < i; ++j) {
+        const R s = R(0.5) * (C[i][j] + C[j][i]);
+        C[i][j] = s;
+        C[j][i] = s;
+      }
+  }
+}
+
+}  // namespace psm

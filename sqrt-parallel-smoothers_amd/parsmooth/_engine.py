@@ -1,0 +1,149 @@
+"""Whole-pass driver over a ragged batch of sequences: shapes and dtypes are
+checked here, buffers are torch tensors on the HIP device, and every pass is
+one call into libparsmooth_hip.so."""
+from __future__ import annotations
+
+import ctypes
+from typing import Optional, Sequence
+
+import numpy as np
+import torch
+
+from . import _native as N
+from ._base import MVNSqrt, MVNStandard
+from .linearization._methods import Method, model_struct
+
+
+def to_dev(x, dtype, dev):
+    if isinstance(x, torch.Tensor):
+        return x.to(device=dev, dtype=dtype).contiguous()
+    return torch.as_tensor(np.asarray(x, dtype=np.float64)).to(device=dev, dtype=dtype).contiguous()
+
+
+def compute_dtype(x0) -> torch.dtype:
+    m = x0[0]
+    if isinstance(m, torch.Tensor) and m.dtype in (torch.float32, torch.float64):
+        return m.dtype
+    if isinstance(m, np.ndarray) and m.dtype == np.float32:
+        return torch.float32
+    return torch.float64
+
+
+class Problem:
+    """Dims, dtype, form and the sequence layout of one batch.
+
+    ``lengths[s]`` is the number of observations of sequence s; it owns
+    ``lengths[s] + 1`` states (its prior state first)."""
+
+    def __init__(self, nx: int, ny: int, lengths: Sequence[int], dtype: torch.dtype, sqrt: bool,
+                 dev, chunk: int = 0):
+        lengths = np.asarray(lengths, dtype=np.int64).reshape(-1)
+        if lengths.size == 0 or np.any(lengths < 1):
+            raise ValueError("every sequence needs at least one observation")
+        self.nx, self.ny = int(nx), int(ny)
+        self.lengths = lengths
+        self.n_seq = int(lengths.size)
+        offs = np.zeros(self.n_seq + 1, dtype=np.int64)
+        offs[1:] = np.cumsum(lengths + 1)
+        self.state_offsets_host = offs
+        self.n_states = int(offs[-1])
+        self.n_obs = int(lengths.sum())
+        self.dtype = dtype
+        self.sqrt = sqrt
+        self.dev = dev
+        self.state_offsets = torch.as_tensor(offs).to(dev)
+        if not N.lib().psm_supported(N.dtype_code(dtype), N.FORM_SQRT if sqrt else N.FORM_STD,
+                                     self.nx, self.ny):
+            raise NotImplementedError(f"no compiled kernels for nx={nx}, ny={ny}, dtype={dtype}")
+        self.struct = N.PsmProblem(N.dtype_code(dtype), N.FORM_SQRT if sqrt else N.FORM_STD,
+                                   self.nx, self.ny, self.n_seq, int(chunk), self.n_states,
+                                   self.state_offsets.data_ptr())
+        self.ws_bytes = int(N.lib().psm_workspace_bytes(ctypes.byref(self.struct)))
+        if self.ws_bytes == 0:
+            N.check(-1, "psm_workspace_bytes")
+        self._ws = None
+
+    @property
+    def workspace(self) -> torch.Tensor:
+        if self._ws is None:
+            self._ws = torch.empty(self.ws_bytes, dtype=torch.uint8, device=self.dev)
+        return self._ws
+
+    def new_traj(self):
+        m = torch.empty((self.n_states, self.nx), dtype=self.dtype, device=self.dev)
+        L = torch.empty((self.n_states, self.nx, self.nx), dtype=self.dtype, device=self.dev)
+        return m, L
+
+    def default_nominal(self):
+        """Zero mean and identity chol/cov at every state
+        (parsmooth/parallel/_filtering.py:25-28, _smoothing.py:21-28)."""
+        m = torch.zeros((self.n_states, self.nx), dtype=self.dtype, device=self.dev)
+        L = torch.eye(self.nx, dtype=self.dtype, device=self.dev).expand(self.n_states, self.nx, self.nx).contiguous()
+        return m, L
+
+
+class Models:
+    """ctypes views of the transition / observation models and the sigma table."""
+
+    def __init__(self, prob: Problem, trans, obs, method: Method):
+        self.trans, keep_t = model_struct(trans, prob.dtype, prob.dev, prob.sqrt)
+        self.obs = None
+        keep_o = ()
+        if obs is not None:
+            self.obs, keep_o = model_struct(obs, prob.dtype, prob.dev, prob.sqrt)
+            if obs.function.nin != prob.nx or obs.function.nout != prob.ny:
+                raise ValueError("observation function dims do not match the problem")
+        if trans.function.nin != prob.nx or trans.function.nout != prob.nx:
+            raise ValueError("transition function dims do not match the state dimension")
+        self.sigma, keep_s = method.sigma(prob.nx, prob.dtype, prob.dev)
+        self._keep = (keep_t, keep_o, keep_s)
+
+    def sigma_ref(self):
+        return ctypes.byref(self.sigma) if self.sigma is not None else None
+
+
+def filter_pass(prob: Problem, mods: Models, ys, m0, L0, nom_m, nom_L, want_ell: bool,
+                out=None):
+    out_m, out_L = out if out is not None else prob.new_traj()
+    ell = torch.empty(prob.n_seq, dtype=torch.float64, device=prob.dev) if want_ell else None
+    rc = N.lib().psm_filter(ctypes.byref(prob.struct), ctypes.byref(mods.trans), ctypes.byref(mods.obs),
+                            mods.sigma_ref(), N.ptr(ys), N.ptr(m0), N.ptr(L0), N.ptr(nom_m),
+                            N.ptr(nom_L), N.ptr(out_m), N.ptr(out_L), N.ptr(ell),
+                            N.ptr(prob.workspace), prob.ws_bytes, N.stream_ptr())
+    N.check(rc, "psm_filter")
+    return out_m, out_L, ell
+
+
+def smoother_pass(prob: Problem, mods: Models, fm, fL, nom_m, nom_L, out=None):
+    out_m, out_L = out if out is not None else prob.new_traj()
+    rc = N.lib().psm_smoother(ctypes.byref(prob.struct), ctypes.byref(mods.trans), mods.sigma_ref(),
+                              N.ptr(fm), N.ptr(fL), N.ptr(nom_m), N.ptr(nom_L), N.ptr(out_m),
+                              N.ptr(out_L), N.ptr(prob.workspace), prob.ws_bytes, N.stream_ptr())
+    N.check(rc, "psm_smoother")
+    return out_m, out_L
+
+
+def filter_smoother_pass(prob: Problem, mods: Models, ys, m0, L0, nom_m, nom_L, filt, out,
+                         use_graph: bool = False):
+    fm, fL = filt
+    om, oL = out
+    rc = N.lib().psm_filter_smoother(ctypes.byref(prob.struct), ctypes.byref(mods.trans),
+                                     ctypes.byref(mods.obs), mods.sigma_ref(), N.ptr(ys), N.ptr(m0),
+                                     N.ptr(L0), N.ptr(nom_m), N.ptr(nom_L), N.ptr(fm), N.ptr(fL),
+                                     N.ptr(om), N.ptr(oL), N.ptr(prob.workspace), prob.ws_bytes,
+                                     int(bool(use_graph)), N.stream_ptr())
+    N.check(rc, "psm_filter_smoother")
+    return om, oL
+
+
+def mean_sq_diff(a: torch.Tensor, b: torch.Tensor) -> torch.Tensor:
+    """mean((a - b)^2) on the device, accumulated in FP64 (one-element tensor)."""
+    out = torch.empty(1, dtype=torch.float64, device=a.device)
+    a = a.contiguous()
+    b = b.to(a.dtype).contiguous()
+    if a.shape != b.shape:
+        raise ValueError("mean_sq_diff needs equal shapes")
+    rc = N.lib().psm_mean_sq_diff(N.dtype_code(a.dtype), a.numel(), N.ptr(a), N.ptr(b), N.ptr(out),
+                                  N.stream_ptr())
+    N.check(rc, "psm_mean_sq_diff")
+    return out

@@ -1,0 +1,126 @@
+"""Ragged batches: many independent sequences of uneven length smoothed by one
+launch sequence per pass.
+
+The reference handles one sequence per call; here sequences are concatenated
+(sequence s owns states [state_offsets[s], state_offsets[s+1]), its prior
+state first) and the filter / smoother scans run over the concatenation in
+single launches.  ``IteratedSmoother`` owns every device buffer, so repeated
+runs reuse them and each pass can be replayed from a captured hipGraph.
+"""
+from __future__ import annotations
+
+from typing import Callable, Optional, Sequence
+
+import numpy as np
+import torch
+
+from . import _native as N
+from ._base import MVNSqrt, MVNStandard
+from ._engine import Models, Problem, filter_pass, filter_smoother_pass, to_dev
+from .linearization._methods import resolve
+
+
+class IteratedSmoother:
+    """Iterated (IEKS / IPLS) parallel smoother for a ragged batch.
+
+    Parameters
+    ----------
+    lengths: number of observations of each sequence.
+    transition_model, observation_model: FunctionalModel (device functions).
+    linearization_method: parsmooth.linearization.extended / cubature / ...
+    dtype: torch.float64 or torch.float32.
+    sqrt: square-root (MVNSqrt) or covariance (MVNStandard) form.
+    """
+
+    def __init__(self, lengths: Sequence[int], transition_model, observation_model,
+                 linearization_method: Callable, dtype=torch.float64, sqrt: bool = True, chunk: int = 0):
+        self.dev = N.device()
+        self.dtype = dtype
+        self.sqrt = sqrt
+        nx = transition_model.function.nin
+        ny = observation_model.function.nout
+        self.prob = Problem(nx, ny, lengths, dtype, sqrt, self.dev, chunk)
+        self.mods = Models(self.prob, transition_model, observation_model, resolve(linearization_method))
+        self.nx, self.ny = nx, ny
+        self._bufs = [self.prob.new_traj(), self.prob.new_traj()]
+        self._filt = self.prob.new_traj()
+        self._default = self.prob.default_nominal()
+        self._init = self.prob.new_traj()
+        self.ys = None
+        self.m0 = None
+        self.L0 = None
+
+    @property
+    def state_offsets(self) -> np.ndarray:
+        return self.prob.state_offsets_host
+
+    def set_data(self, ys, m0, L0):
+        """ys: (n_obs, ny) concatenated observations; m0: (n_seq, nx);
+        L0: (n_seq, nx, nx) prior chol (sqrt form) or covariance."""
+        p = self.prob
+        self.ys = to_dev(ys, self.dtype, self.dev).reshape(p.n_obs, self.ny)
+        self.m0 = to_dev(m0, self.dtype, self.dev).reshape(p.n_seq, self.nx)
+        self.L0 = to_dev(L0, self.dtype, self.dev).reshape(p.n_seq, self.nx, self.nx)
+
+    def pass_once(self, nominal, out, use_graph=False):
+        filter_smoother_pass(self.prob, self.mods, self.ys, self.m0, self.L0, nominal[0], nominal[1],
+                             self._filt, out, use_graph)
+        return out
+
+    def run(self, n_iter: int, init_nominal=None, use_graph: bool = True,
+            return_loglikelihood: bool = False):
+        """Same passes as ``iterated_smoothing(..., init_nominal, criterion=lambda i, *_: i < n_iter)``:
+        one pass at the default nominal when ``init_nominal`` is None, then
+        ``n_iter`` passes.  Returns (mean, chol) of all states (views of internal
+        buffers, valid until the next run) and optionally the per-sequence
+        log-likelihood at the final nominal trajectory."""
+        if self.ys is None:
+            raise RuntimeError("call set_data first")
+        if init_nominal is None:
+            cur = self.pass_once(self._default, self._bufs[0], use_graph)
+            k = 1
+        else:
+            self._init[0].copy_(to_dev(init_nominal[0], self.dtype, self.dev).reshape(self._init[0].shape))
+            self._init[1].copy_(to_dev(init_nominal[1], self.dtype, self.dev).reshape(self._init[1].shape))
+            cur = self._init
+            k = 0
+        for _ in range(int(n_iter)):
+            nxt = self._bufs[k % 2]
+            cur = self.pass_once(cur, nxt, use_graph)
+            k += 1
+        if return_loglikelihood:
+            _, _, ell = filter_pass(self.prob, self.mods, self.ys, self.m0, self.L0, cur[0], cur[1], True,
+                                    out=self._filt)
+            return cur[0], cur[1], ell
+        return cur[0], cur[1]
+
+    def split(self, traj: torch.Tensor):
+        """Per-sequence views of a (n_states, ...) trajectory tensor."""
+        o = self.prob.state_offsets_host
+        return [traj[o[s]:o[s + 1]] for s in range(self.prob.n_seq)]
+
+
+def iterated_smoothing(observations: Sequence, x0, transition_model, observation_model,
+                       linearization_method: Callable, n_iter: int, init_nominal=None,
+                       return_loglikelihood: bool = False):
+    """Batched iterated smoothing with a fixed iteration count.
+
+    observations: list of (T_s, ny) arrays; x0: MVNSqrt / MVNStandard with
+    leading axis n_seq.  Returns a list of per-sequence trajectories of the same
+    type as x0 (and the per-sequence log-likelihoods)."""
+    sqrt = isinstance(x0, MVNSqrt)
+    m0 = x0[0]
+    dtype = m0.dtype if isinstance(m0, torch.Tensor) and m0.dtype in (torch.float32, torch.float64) \
+        else torch.float64
+    lengths = [int(np.shape(y)[0]) for y in observations]
+    runner = IteratedSmoother(lengths, transition_model, observation_model, linearization_method,
+                              dtype=dtype, sqrt=sqrt)
+    dev = runner.dev
+    ys = torch.cat([to_dev(y, dtype, dev).reshape(-1, runner.ny) for y in observations])
+    runner.set_data(ys, x0[0], x0[1])
+    out = runner.run(n_iter, init_nominal, use_graph=False, return_loglikelihood=return_loglikelihood)
+    cls = type(x0)
+    trajs = [cls(m.clone(), L.clone()) for m, L in zip(runner.split(out[0]), runner.split(out[1]))]
+    if return_loglikelihood:
+        return trajs, out[2]
+    return trajs

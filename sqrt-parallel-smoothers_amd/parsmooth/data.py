@@ -1,0 +1,67 @@
+"""Synthetic inputs of the reference's experiments (numpy, seeded).
+
+* ``bearings_data``: coordinated-turn target seen by two bearing sensors,
+  restating notebooks/bearing_data.py:137-198 (float32 simulation; the turn rate
+  follows 1 + q dt cumsum(randn); the state is propagated with the matrix
+  exponential of the 4x4 turn generator).
+* ``lgssm_data``: linear-Gaussian simulation, tests/_lgssm.py:42-95.
+* ``cv_model``: the nx=4 constant-velocity tracking model (white acceleration).
+"""
+from __future__ import annotations
+
+import numpy as np
+import scipy.linalg as linalg
+
+
+def bearings_data(x0, dt, r, T, s1, s2, q=10.0, random_state=None):
+    """Returns (ts, true_states (T+1, 5), observations (T, 2)), all float32."""
+    if random_state is None or isinstance(random_state, (int, np.integer)):
+        random_state = np.random.RandomState(random_state)
+    a_s = (1 + q * dt * np.cumsum(random_state.randn(T))).astype(np.float32)
+    s1 = np.asarray(s1, dtype=np.float32)
+    s2 = np.asarray(s2, dtype=np.float32)
+    x = np.copy(x0).astype(np.float32)
+    observations = np.empty((T, 2), dtype=np.float32)
+    true_states = np.zeros((T + 1, 5), dtype=np.float32)
+    ts = np.linspace(dt, (T + 1) * dt, T).astype(np.float32)
+    true_states[0, :4] = x
+    normals = random_state.randn(T, 2).astype(np.float32)
+    for i, a in enumerate(a_s):
+        G = np.array([[0, 0, 1, 0], [0, 0, 0, 1], [0, 0, 0, a], [0, 0, -a, 0]], dtype=np.float32)
+        x = linalg.expm(G * dt) @ x
+        y1 = np.arctan2(x[1] - s1[1], x[0] - s1[0]) + r * normals[i, 0]
+        y2 = np.arctan2(x[1] - s2[1], x[0] - s2[0]) + r * normals[i, 1]
+        observations[i] = [y1, y2]
+        true_states[i + 1] = np.concatenate((x, np.array([a])))
+    return ts, true_states, observations
+
+
+def lgssm_data(x0, A, H, R, Q, b, c, T, random_state=None, chol_R=None):
+    """Returns (true_states (T+1, nx), observations (T, ny)), float32."""
+    if random_state is None or isinstance(random_state, (int, np.integer)):
+        random_state = np.random.RandomState(random_state)
+    nq, nr = Q.shape[0], R.shape[0]
+    normals = random_state.randn(T, nq + nr).astype(np.float32)
+    if chol_R is None:
+        chol_R = np.linalg.cholesky(R)
+    chol_Q = np.linalg.cholesky(Q)
+    x = np.copy(x0).astype(np.float32)
+    obs = np.empty((T, nr), dtype=np.float32)
+    states = np.empty((T + 1, nq), dtype=np.float32)
+    states[0] = x
+    for i in range(T):
+        x = A @ x + chol_Q @ normals[i, :nq] + b
+        states[i + 1] = x
+        obs[i] = H @ x + chol_R @ normals[i, nq:] + c
+    return states, obs
+
+
+def cv_model(dt=0.1, q=1.0, r=0.5):
+    """Constant-velocity model on (x, y, vx, vy): F, Q (white acceleration of
+    intensity q), H = [I 0], R = r^2 I."""
+    F = np.array([[1, 0, dt, 0], [0, 1, 0, dt], [0, 0, 1, 0], [0, 0, 0, 1]], dtype=np.float64)
+    Q = q * np.array([[dt ** 3 / 3, 0, dt ** 2 / 2, 0], [0, dt ** 3 / 3, 0, dt ** 2 / 2],
+                      [dt ** 2 / 2, 0, dt, 0], [0, dt ** 2 / 2, 0, dt]])
+    H = np.eye(2, 4)
+    R = r ** 2 * np.eye(2)
+    return F, Q, H, R

@@ -1,0 +1,167 @@
+"""Device-executable model functions for ``FunctionalModel.function``.
+
+A ``FunctionalModel`` in the reference holds an arbitrary JAX callable.  The
+HIP kernels evaluate the model (and its Jacobian) per time step on the GPU, so
+the function must be one the library knows how to evaluate there.  Each class
+below is callable on torch tensors or numpy arrays (leading batch axes allowed)
+for convenience, and carries the identifiers the kernels need.
+
+* ``LinearFunction(A)``: x -> A x            (tests/_lgssm.py:5-39)
+* ``CoordinatedTurnTransition(dt)``          (tests/bearings/bearings_utils.py:7-44)
+* ``BearingsObservation(s1, s2)``            (tests/bearings/bearings_utils.py:47-69)
+* ``Elementwise("sin"|"cos"|"exp"|"arctan", dim)`` (tests/test_linearization.py:157)
+
+``make_parameters`` mirrors tests/bearings/bearings_utils.py:72-115 and
+notebooks/bearing_data.py:93-134.
+"""
+from __future__ import annotations
+
+import numpy as np
+import torch
+
+from . import _native as N
+
+
+def _as_tensor(x, dtype=None, device=None):
+    if isinstance(x, torch.Tensor):
+        return x.to(dtype=dtype or x.dtype, device=device or x.device)
+    return torch.as_tensor(np.asarray(x), dtype=dtype, device=device)
+
+
+class DeviceFunction:
+    """Base class: ``kind`` (model id of include/psm.h), ``nin``, ``nout`` and the
+    parameter vector the kernels read."""
+
+    kind: int = -1
+    nin: int = 0
+    nout: int = 0
+
+    def __init__(self):
+        self._cache = {}
+
+    def _param_values(self) -> np.ndarray:
+        return np.zeros(1)
+
+    def device_params(self, dtype: torch.dtype, device: torch.device) -> torch.Tensor:
+        key = (dtype, str(device))
+        t = self._cache.get(key)
+        if t is None:
+            t = torch.as_tensor(np.ascontiguousarray(self._param_values(), dtype=np.float64)).to(
+                dtype=dtype, device=device)
+            self._cache[key] = t
+        return t
+
+    def __call__(self, x):  # pragma: no cover - exercised through subclasses
+        raise NotImplementedError
+
+
+class LinearFunction(DeviceFunction):
+    """x -> A x."""
+
+    kind = N.MODEL_LINEAR
+
+    def __init__(self, A):
+        super().__init__()
+        self.A = np.asarray(A.detach().cpu() if isinstance(A, torch.Tensor) else A, dtype=np.float64)
+        if self.A.ndim != 2:
+            raise ValueError("A must be a matrix")
+        self.nout, self.nin = self.A.shape
+
+    def _param_values(self):
+        return self.A.reshape(-1)
+
+    def __call__(self, x):
+        if isinstance(x, torch.Tensor):
+            A = torch.as_tensor(self.A, dtype=x.dtype, device=x.device)
+            return x @ A.T
+        return np.asarray(x) @ self.A.T
+
+
+class CoordinatedTurnTransition(DeviceFunction):
+    """Coordinated-turn transition on (x, y, vx, vy, w) with step dt; at
+    |w| < 1e-6 the terms sin(w dt)/w and (cos(w dt) - 1)/w are replaced by dt and
+    0 (tests/bearings/bearings_utils.py:24-37)."""
+
+    kind = N.MODEL_COORDINATED_TURN
+    nin = 5
+    nout = 5
+
+    def __init__(self, dt):
+        super().__init__()
+        self.dt = float(dt)
+
+    def _param_values(self):
+        return np.array([self.dt])
+
+    def __call__(self, x):
+        lib = torch if isinstance(x, torch.Tensor) else np
+        w = x[..., 4]
+        c = lib.cos(w * self.dt)
+        s = lib.sin(w * self.dt)
+        small = lib.abs(w) < 1e-6
+        safe_w = lib.where(small, lib.ones_like(w), w)
+        a = lib.where(small, self.dt * lib.ones_like(w), s / safe_w)
+        b = lib.where(small, lib.zeros_like(w), (c - 1.0) / safe_w)
+        out = [x[..., 0] + a * x[..., 2] - b * x[..., 3],
+               x[..., 1] + b * x[..., 2] + a * x[..., 3],
+               c * x[..., 2] + s * x[..., 3],
+               -s * x[..., 2] + c * x[..., 3],
+               x[..., 4]]
+        return lib.stack(out, -1) if lib is torch else np.stack(out, -1)
+
+
+class BearingsObservation(DeviceFunction):
+    """Angles (arctan2) of the position (x[0], x[1]) seen from two sensors, with
+    no angle wrapping."""
+
+    kind = N.MODEL_BEARINGS
+    nout = 2
+
+    def __init__(self, s1, s2, nin: int = 5):
+        super().__init__()
+        self.s1 = np.asarray(s1, dtype=np.float64).reshape(2)
+        self.s2 = np.asarray(s2, dtype=np.float64).reshape(2)
+        self.nin = int(nin)
+
+    def _param_values(self):
+        return np.concatenate([self.s1, self.s2])
+
+    def __call__(self, x):
+        lib = torch if isinstance(x, torch.Tensor) else np
+        out = [lib.arctan2(x[..., 1] - self.s1[1], x[..., 0] - self.s1[0]),
+               lib.arctan2(x[..., 1] - self.s2[1], x[..., 0] - self.s2[0])]
+        return lib.stack(out, -1) if lib is torch else np.stack(out, -1)
+
+
+class Elementwise(DeviceFunction):
+    """Elementwise sin / cos / exp / arctan on R^dim."""
+
+    _KINDS = {"sin": N.MODEL_SIN, "cos": N.MODEL_COS, "exp": N.MODEL_EXP, "arctan": N.MODEL_ARCTAN}
+
+    def __init__(self, name: str, dim: int):
+        super().__init__()
+        if name not in self._KINDS:
+            raise ValueError(f"unknown elementwise function {name!r}")
+        self.name = name
+        self.kind = self._KINDS[name]
+        self.nin = self.nout = int(dim)
+
+    def __call__(self, x):
+        lib = torch if isinstance(x, torch.Tensor) else np
+        return {"sin": lib.sin, "cos": lib.cos, "exp": lib.exp, "arctan": lib.arctan}[self.name](x)
+
+
+def make_parameters(qc, qw, r, dt, s1, s2):
+    """Q, R and the two model functions of the coordinated-turn bearings-only
+    problem (tests/bearings/bearings_utils.py:72-115)."""
+    Q = np.array([[qc * dt ** 3 / 3, 0, qc * dt ** 2 / 2, 0, 0],
+                  [0, qc * dt ** 3 / 3, 0, qc * dt ** 2 / 2, 0],
+                  [qc * dt ** 2 / 2, 0, qc * dt, 0, 0],
+                  [0, qc * dt ** 2 / 2, 0, qc * dt, 0],
+                  [0, 0, 0, 0, dt * qw]])
+    R = r ** 2 * np.eye(2)
+    return Q, R, BearingsObservation(s1, s2), CoordinatedTurnTransition(dt)
+
+
+def is_device_function(f) -> bool:
+    return isinstance(f, DeviceFunction)

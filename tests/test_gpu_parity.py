@@ -1,0 +1,396 @@
+"""Parity of the HIP path (through the C ABI via parsmooth) with the FP64 oracle.
+
+FP64 tolerance: 1e-9 relative to max(1, |reference|) for means, gains and
+covariances (L L^T); chol factors are compared directly as well (both sides
+return the non-negative-diagonal factor).  FP32 tolerances are stated per test.
+"""
+import functools
+
+import numpy as np
+import pytest
+import torch
+
+from oracle import psm_oracle as O
+
+pytestmark = pytest.mark.gpu
+
+if not torch.cuda.is_available():  # pragma: no cover
+    pytest.skip("needs a HIP device", allow_module_level=True)
+
+import parsmooth  # noqa: E402
+from parsmooth import FunctionalModel, MVNSqrt, MVNStandard  # noqa: E402
+from parsmooth import models as M  # noqa: E402
+from parsmooth.linearization import cubature, extended, gauss_hermite, unscented  # noqa: E402
+from parsmooth.parallel import _operators as ops  # noqa: E402
+from parsmooth.parallel._filtering import _sqrt_associative_params_one, _standard_associative_params_one  # noqa: E402
+from parsmooth.parallel._smoothing import _sqrt_associative_params, _standard_associative_params  # noqa: E402
+
+DEV = torch.device("cuda", 0)
+TOL64 = 1e-9
+
+
+def t64(a):
+    return torch.as_tensor(np.asarray(a, dtype=np.float64), device=DEV)
+
+
+def npy(x):
+    return x.detach().cpu().numpy().astype(np.float64)
+
+
+def close(a, b, tol=TOL64, what=""):
+    a = np.asarray(a, dtype=np.float64)
+    b = np.asarray(b, dtype=np.float64)
+    scale = max(1.0, float(np.abs(b).max()) if b.size else 1.0)
+    err = float(np.abs(a - b).max()) / scale if a.size else 0.0
+    assert err <= tol, f"{what}: rel err {err:.3e} > {tol:.1e}"
+
+
+def gram(L):
+    L = np.asarray(L)
+    return L @ np.swapaxes(L, -1, -2)
+
+
+def rand_lower(n, rng):
+    L = rng.rand(n, n)
+    L[np.triu_indices(n, 1)] = 0.0
+    return L
+
+
+OMETH = {"extended": O.extended, "cubature": O.cubature, "gauss_hermite": O.gauss_hermite,
+         "unscented": O.unscented}
+DMETH = {"extended": extended, "cubature": cubature, "gauss_hermite": gauss_hermite, "unscented": unscented}
+
+
+# ---------------------------------------------------------------- operators
+@pytest.mark.parametrize("dim", [1, 2, 3, 4, 5])
+@pytest.mark.parametrize("seed", [0, 42])
+@pytest.mark.parametrize("sqrt", [True, False])
+def test_filtering_operator(dim, seed, sqrt):
+    rng = np.random.RandomState(seed)
+    B = 7
+    def elem():
+        U = np.stack([rand_lower(dim, rng) for _ in range(B)])
+        Z = np.stack([rand_lower(dim, rng) for _ in range(B)])
+        e = [rng.randn(B, dim, dim), rng.randn(B, dim), U, rng.randn(B, dim), Z]
+        if not sqrt:
+            e[2], e[4] = gram(U), gram(Z)
+        return e
+    e1, e2 = elem(), elem()
+    ref = (O.sqrt_filtering_operator if sqrt else O.standard_filtering_operator)(tuple(e1), tuple(e2))
+    got = (ops.sqrt_filtering_operator if sqrt else ops.standard_filtering_operator)(
+        tuple(t64(x) for x in e1), tuple(t64(x) for x in e2))
+    got = [npy(x) for x in got]
+    for k, name in enumerate("A b U eta Z".split()):
+        close(got[k], ref[k], what=name)
+    if sqrt:
+        close(gram(got[2]), gram(ref[2]), what="UU^T")
+        close(gram(got[4]), gram(ref[4]), what="ZZ^T")
+
+
+@pytest.mark.parametrize("dim", [1, 2, 3, 4, 5])
+@pytest.mark.parametrize("seed", [0, 42])
+@pytest.mark.parametrize("sqrt", [True, False])
+def test_smoothing_operator(dim, seed, sqrt):
+    rng = np.random.RandomState(seed)
+    B = 7
+    def elem():
+        D = np.stack([rand_lower(dim, rng) for _ in range(B)])
+        return [rng.randn(B, dim), rng.randn(B, dim, dim), D if sqrt else gram(D)]
+    e1, e2 = elem(), elem()
+    ref = (O.sqrt_smoothing_operator if sqrt else O.standard_smoothing_operator)(tuple(e1), tuple(e2))
+    got = (ops.sqrt_smoothing_operator if sqrt else ops.standard_smoothing_operator)(
+        tuple(t64(x) for x in e1), tuple(t64(x) for x in e2))
+    got = [npy(x) for x in got]
+    for k, name in enumerate("g E D".split()):
+        close(got[k], ref[k], what=name)
+
+
+# ---------------------------------------------------------------- elements
+def _system(dim_x, dim_y, seed):
+    np.random.seed(seed)
+    x0, cx0, F, Q, cholQ, b, _ = O.get_system(dim_x, dim_x)
+    _, _, H, R, cholR, c, y = O.get_system(dim_x, dim_y)
+    return x0, cx0, F, Q, cholQ, b, H, R, cholR, c, y
+
+
+@pytest.mark.parametrize("dim_x", [1, 2, 3])
+@pytest.mark.parametrize("dim_y", [1, 2, 3])
+@pytest.mark.parametrize("seed", [0, 42])
+@pytest.mark.parametrize("method", ["extended", "cubature"])
+def test_filter_element(dim_x, dim_y, seed, method):
+    """tests/test_parallel_filter.py:26-73, against the oracle."""
+    x0, cx0, F, Q, cholQ, b, H, R, cholR, c, y = _system(dim_x, dim_y, seed)
+    m_nom = np.random.randn(dim_x)
+    for sqrt in (True, False):
+        cls_o, cls_d = (O.MVNSqrt, MVNSqrt) if sqrt else (O.MVNStandard, MVNStandard)
+        qo, ro = (O.MVNSqrt(b, cholQ), O.MVNSqrt(c, cholR)) if sqrt else (O.MVNStandard(b, Q), O.MVNStandard(c, R))
+        qd = (MVNSqrt if sqrt else MVNStandard)(t64(qo[0]), t64(qo[1]))
+        rd = (MVNSqrt if sqrt else MVNStandard)(t64(ro[0]), t64(ro[1]))
+        nom_o = cls_o(m_nom, np.eye(dim_x))
+        Ft, qt, bt = OMETH[method](O.FunctionalModel(O.Linear(F), qo), nom_o)
+        Hh, rr, cc = OMETH[method](O.FunctionalModel(O.Linear(H), ro), nom_o)
+        mx, Px = (cx0.mean, cx0.chol) if sqrt else (x0.mean, x0.cov)
+        ref = (O.sqrt_filter_element if sqrt else O.std_filter_element)(Ft, qt, bt, Hh, rr, cc, mx, Px, y)
+        fn = _sqrt_associative_params_one if sqrt else _standard_associative_params_one
+        nom_d = cls_d(t64(m_nom), t64(np.eye(dim_x)))
+        got, ssm = fn(DMETH[method], FunctionalModel(M.LinearFunction(F), qd),
+                      FunctionalModel(M.LinearFunction(H), rd), nom_d, nom_d, t64(mx), t64(Px), t64(y))
+        got = [npy(g) for g in got]
+        close(got[0], ref[0], what="A")
+        close(got[1], ref[1], what="b")
+        close(got[3], ref[3], what="eta")
+        if sqrt:
+            close(gram(got[2]), gram(ref[2]), what="UU^T")
+            close(gram(got[4]), gram(ref[4]), what="ZZ^T")
+        else:
+            close(got[2], ref[2], what="C")
+            close(got[4], ref[4], what="J")
+
+
+@pytest.mark.parametrize("dim_x", [1, 2, 3, 5])
+@pytest.mark.parametrize("seed", [0, 42])
+def test_smoother_element(dim_x, seed):
+    np.random.seed(seed)
+    x0, cx0, F, Q, cholQ, b, _ = O.get_system(dim_x, dim_x)
+    F = np.random.randn(dim_x, dim_x)
+    nom = np.random.randn(dim_x)
+    for sqrt in (True, False):
+        q = O.MVNSqrt(b, cholQ) if sqrt else O.MVNStandard(b, Q)
+        ref = (O.sqrt_smoother_element if sqrt else O.std_smoother_element)(
+            F, q[1], b, x0.mean, cx0.chol if sqrt else x0.cov)
+        cls = MVNSqrt if sqrt else MVNStandard
+        fn = _sqrt_associative_params if sqrt else _standard_associative_params
+        got = fn(extended, FunctionalModel(M.LinearFunction(F), cls(t64(b), t64(q[1]))),
+                 cls(t64(nom), t64(np.eye(dim_x))), t64(x0.mean), t64(cx0.chol if sqrt else x0.cov))
+        got = [npy(g) for g in got]
+        close(got[0], ref[0], what="g")
+        close(got[1], ref[1], what="E")
+        close(got[2], ref[2], what="D")
+
+
+# ---------------------------------------------------------------- linearization
+def _model_pairs(rng):
+    A = rng.randn(3, 5)
+    return [
+        ("linear", M.LinearFunction(A), O.Linear(A), 5, 3),
+        ("ct", M.CoordinatedTurnTransition(0.01), O.CoordinatedTurn(0.01), 5, 5),
+        ("bearings", M.BearingsObservation((-1.5, 0.5), (1.0, 1.0)), O.Bearings((-1.5, 0.5), (1.0, 1.0)), 5, 2),
+        ("sin", M.Elementwise("sin", 3), O.Elementwise("sin"), 3, 3),
+        ("cos", M.Elementwise("cos", 3), O.Elementwise("cos"), 3, 3),
+        ("exp", M.Elementwise("exp", 2), O.Elementwise("exp"), 2, 2),
+        ("arctan", M.Elementwise("arctan", 3), O.Elementwise("arctan"), 3, 3),
+    ]
+
+
+@pytest.mark.parametrize("method", list(OMETH))
+@pytest.mark.parametrize("sqrt", [True, False])
+def test_linearize(method, sqrt):
+    rng = np.random.RandomState(3)
+    for name, fd, fo, nin, nout in _model_pairs(rng):
+        for w in (0.7, 1e-8):
+            B = 5
+            m = rng.randn(B, nin)
+            if name == "ct":
+                m[:, 4] = w
+            L = np.stack([rand_lower(nin, rng) * 0.3 + 0.1 * np.eye(nin) for _ in range(B)])
+            qm, qc = rng.randn(nout), rand_lower(nout, rng)
+            if sqrt:
+                xo, qo = O.MVNSqrt(m, L), O.MVNSqrt(qm, qc)
+                xd, qd = MVNSqrt(t64(m), t64(L)), MVNSqrt(t64(qm), t64(qc))
+            else:
+                xo, qo = O.MVNStandard(m, gram(L)), O.MVNStandard(qm, gram(qc))
+                xd, qd = MVNStandard(t64(m), t64(gram(L))), MVNStandard(t64(qm), t64(gram(qc)))
+            ref = OMETH[method](O.FunctionalModel(fo, qo), xo)
+            got = [npy(g) for g in DMETH[method](FunctionalModel(fd, qd), xd)]
+            close(got[0], ref[0], tol=1e-8, what=f"{name} F")
+            close(got[2], ref[2], tol=1e-8, what=f"{name} b")
+            close(gram(got[1]) if sqrt else got[1], gram(ref[1]) if sqrt else ref[1], tol=1e-8,
+                  what=f"{name} Q")
+
+
+def test_gauss_hermite_order_partial():
+    rng = np.random.RandomState(0)
+    m, L = rng.randn(2, 3), np.stack([np.eye(3)] * 2)
+    A = rng.randn(3, 3)
+    for order in (2, 4):
+        ref = O.gauss_hermite(O.FunctionalModel(O.Elementwise("sin"), O.MVNSqrt(np.zeros(3), np.eye(3))),
+                              O.MVNSqrt(m, L), order=order)
+        got = functools.partial(gauss_hermite, order=order)(
+            FunctionalModel(M.Elementwise("sin", 3), MVNSqrt(t64(np.zeros(3)), t64(np.eye(3)))),
+            MVNSqrt(t64(m), t64(L)))
+        close(npy(got[0]), ref[0], tol=1e-9, what="F")
+
+
+# ---------------------------------------------------------------- filter / smoother / iterated
+def _linear_problem(dim_x, dim_y, seed, T):
+    np.random.seed(seed)
+    x0, cx0, F, Q, cholQ, b, _ = O.get_system(dim_x, dim_x)
+    F = np.eye(dim_x) + 0.1 * np.random.randn(dim_x, dim_x)
+    _, _, H, R, cholR, c, _ = O.get_system(dim_x, dim_y)
+    _, ys = O.lgssm_data(x0.mean, F, H, R, Q, b, c, T, random_state=seed)
+    return x0, cx0, F, Q, cholQ, b, H, R, cholR, c, ys
+
+
+@pytest.mark.parametrize("dim_x", [1, 2, 3, 4])
+@pytest.mark.parametrize("dim_y", [1, 2, 3])
+@pytest.mark.parametrize("sqrt", [True, False])
+@pytest.mark.parametrize("T", [1, 10, 333])
+def test_filter_smoother_linear(dim_x, dim_y, sqrt, T):
+    x0, cx0, F, Q, cholQ, b, H, R, cholR, c, ys = _linear_problem(dim_x, dim_y, 1, T)
+    if sqrt:
+        xo, to, oo = cx0, O.FunctionalModel(O.Linear(F), O.MVNSqrt(b, cholQ)), O.FunctionalModel(O.Linear(H), O.MVNSqrt(c, cholR))
+        cls, qn, rn = MVNSqrt, (b, cholQ), (c, cholR)
+    else:
+        xo, to, oo = x0, O.FunctionalModel(O.Linear(F), O.MVNStandard(b, Q)), O.FunctionalModel(O.Linear(H), O.MVNStandard(c, R))
+        cls, qn, rn = MVNStandard, (b, Q), (c, R)
+    nom = np.random.RandomState(2).randn(T + 1, dim_x)
+    nom_o = type(xo)(nom, np.repeat(np.eye(dim_x)[None], T + 1, 0))
+    fo, ell_o = O.filtering(ys, xo, to, oo, O.extended, nom_o, True, return_loglikelihood=True)
+    so = O.smoothing(to, fo, O.extended, nom_o, True)
+    td = FunctionalModel(M.LinearFunction(F), cls(t64(qn[0]), t64(qn[1])))
+    od = FunctionalModel(M.LinearFunction(H), cls(t64(rn[0]), t64(rn[1])))
+    xd = cls(t64(xo[0]), t64(xo[1]))
+    nom_d = cls(t64(nom_o[0]), t64(nom_o[1]))
+    fd, ell_d = parsmooth.filtering(t64(ys), xd, td, od, extended, nom_d, return_loglikelihood=True)
+    close(npy(fd[0]), fo[0], what="filtered mean")
+    close(gram(npy(fd[1])) if sqrt else npy(fd[1]), gram(fo[1]) if sqrt else fo[1], what="filtered cov")
+    assert abs(float(ell_d) - ell_o) <= TOL64 * max(1.0, abs(ell_o)), (float(ell_d), ell_o)
+    sd = parsmooth.smoothing(td, fd, extended, nom_d)
+    close(npy(sd[0]), so[0], what="smoothed mean")
+    close(gram(npy(sd[1])) if sqrt else npy(sd[1]), gram(so[1]) if sqrt else so[1], what="smoothed cov")
+    if sqrt:
+        close(npy(sd[1]), so[1], tol=1e-8, what="smoothed chol")
+
+
+def _bearings_setup(T, seed=0):
+    Q, R, obs_f, trans_f = M.make_parameters(0.01, 0.1, 0.5, 0.01, (-1.5, 0.5), (1.0, 1.0))
+    from parsmooth.data import bearings_data
+    _, _, ys = bearings_data(np.array([0.1, 0.2, 1.0, 0.0]), 0.01, 0.5, T, (-1.5, 0.5), (1.0, 1.0),
+                             random_state=seed)
+    cQ, cR = np.linalg.cholesky(Q), np.linalg.cholesky(R)
+    to = O.FunctionalModel(O.CoordinatedTurn(0.01), O.MVNSqrt(np.zeros(5), cQ))
+    oo = O.FunctionalModel(O.Bearings((-1.5, 0.5), (1.0, 1.0)), O.MVNSqrt(np.zeros(2), cR))
+    td = FunctionalModel(trans_f, MVNSqrt(t64(np.zeros(5)), t64(cQ)))
+    od = FunctionalModel(obs_f, MVNSqrt(t64(np.zeros(2)), t64(cR)))
+    return ys, (to, oo), (td, od)
+
+
+@pytest.mark.parametrize("method", ["extended", "cubature"])
+def test_iterated_bearings_vs_oracle(method):
+    T = 300
+    ys, (to, oo), (td, od) = _bearings_setup(T)
+    m0 = np.array([0.1, 0.2, 1.0, 0.0, 0.0])
+    ref, ell_o = O.iterated_smoothing(ys, O.MVNSqrt(m0, np.eye(5)), to, oo, OMETH[method], None, True,
+                                      criterion=lambda i, *_: i < 3, return_loglikelihood=True)
+    got, ell_d = parsmooth.iterated_smoothing(t64(ys), MVNSqrt(t64(m0), t64(np.eye(5))), td, od,
+                                              DMETH[method], criterion=lambda i, *_: i < 3,
+                                              return_loglikelihood=True)
+    # iterated nonlinear smoothing amplifies rounding differences between passes
+    close(npy(got.mean), ref.mean, tol=1e-8, what="mean")
+    close(gram(npy(got.chol)), gram(ref.chol), tol=1e-8, what="cov")
+    assert abs(float(ell_d) - ell_o) <= 1e-8 * abs(ell_o)
+
+
+@pytest.mark.parametrize("method", ["extended", "cubature"])
+def test_single_pass_bearings_1e9(method):
+    """One filter_smoother pass at an identical nominal: 1e-9 relative."""
+    T = 2000
+    ys, (to, oo), (td, od) = _bearings_setup(T, seed=3)
+    rng = np.random.RandomState(0)
+    nom_m = np.cumsum(0.01 * rng.randn(T + 1, 5), 0) + np.array([0.1, 0.2, 1.0, 0.0, 0.5])
+    nom_c = np.repeat(np.eye(5)[None] * 0.1, T + 1, 0)
+    m0 = np.array([0.1, 0.2, 1.0, 0.0, 0.0])
+    ref = O.filter_smoother(ys, O.MVNSqrt(m0, np.eye(5)), to, oo, OMETH[method], O.MVNSqrt(nom_m, nom_c), True)
+    got = parsmooth.filter_smoother(t64(ys), MVNSqrt(t64(m0), t64(np.eye(5))), td, od, DMETH[method],
+                                    MVNSqrt(t64(nom_m), t64(nom_c)))
+    close(npy(got.mean), ref.mean, what="mean")
+    close(gram(npy(got.chol)), gram(ref.chol), what="cov")
+
+
+def test_ieks_fixture(golden_dir):
+    """tests/test_bearings_only.py:21-72 on the device (extended, 100 iterations)."""
+    ys = np.load(f"{golden_dir}/bearings_ys.npy")
+    exp = np.load(f"{golden_dir}/bearings_ieks.npz")
+    _, (to, oo), (td, od) = _bearings_setup(10)
+    got = parsmooth.iterated_smoothing(t64(ys), MVNSqrt(t64(np.array([-1.0, -1.0, 0, 0, 0])), t64(np.eye(5))),
+                                       td, od, extended, criterion=lambda i, *_: i < 100)
+    np.testing.assert_array_almost_equal(npy(got.mean)[1:], exp["arr_0"], decimal=3)
+    np.testing.assert_array_almost_equal(gram(npy(got.chol))[1:], exp["arr_1"], decimal=3)
+
+
+def test_default_criterion_stops_like_oracle():
+    T = 200
+    ys, (to, oo), (td, od) = _bearings_setup(T, seed=5)
+    m0 = np.array([0.1, 0.2, 1.0, 0.0, 0.0])
+    n_o = []
+    def crit_o(i, p, c):
+        n_o.append(i)
+        return O.default_criterion(i, p, c)
+    n_d = []
+    def crit_d(i, p, c):
+        n_d.append(i)
+        return parsmooth.methods._default_criterion(i, p, c)
+    O.iterated_smoothing(ys, O.MVNSqrt(m0, np.eye(5)), to, oo, O.extended, None, True, criterion=crit_o)
+    parsmooth.iterated_smoothing(t64(ys), MVNSqrt(t64(m0), t64(np.eye(5))), td, od, extended, criterion=crit_d)
+    assert n_o == n_d
+
+
+# ---------------------------------------------------------------- ragged batch
+@pytest.mark.parametrize("method", ["extended", "cubature"])
+def test_ragged_batch_matches_single_sequences(method):
+    from parsmooth.batch import iterated_smoothing as batch_is
+    lengths = [1, 7, 64, 5, 130, 2]
+    rng = np.random.RandomState(11)
+    Q, R, obs_f, trans_f = M.make_parameters(0.01, 0.1, 0.5, 0.01, (-1.5, 0.5), (1.0, 1.0))
+    from parsmooth.data import bearings_data
+    ys = [bearings_data(np.array([0.1, 0.2, 1.0, 0.0]), 0.01, 0.5, T, (-1.5, 0.5), (1.0, 1.0),
+                        random_state=k)[2] for k, T in enumerate(lengths)]
+    m0 = np.array([0.1, 0.2, 1.0, 0.0, 0.0]) + 0.3 * rng.randn(len(lengths), 5)
+    L0 = np.repeat(np.eye(5)[None], len(lengths), 0)
+    cQ, cR = np.linalg.cholesky(Q), np.linalg.cholesky(R)
+    td = FunctionalModel(trans_f, MVNSqrt(t64(np.zeros(5)), t64(cQ)))
+    od = FunctionalModel(obs_f, MVNSqrt(t64(np.zeros(2)), t64(cR)))
+    trajs, ell = batch_is(ys, MVNSqrt(t64(m0), t64(L0)), td, od, DMETH[method], n_iter=2,
+                          return_loglikelihood=True)
+    to = O.FunctionalModel(O.CoordinatedTurn(0.01), O.MVNSqrt(np.zeros(5), cQ))
+    oo = O.FunctionalModel(O.Bearings((-1.5, 0.5), (1.0, 1.0)), O.MVNSqrt(np.zeros(2), cR))
+    ell = npy(ell)
+    for s, y in enumerate(ys):
+        ref, e = O.iterated_smoothing(y, O.MVNSqrt(m0[s], L0[s]), to, oo, OMETH[method], None, True,
+                                      criterion=lambda i, *_: i < 2, return_loglikelihood=True)
+        close(npy(trajs[s].mean), ref.mean, tol=1e-8, what=f"seq {s} mean")
+        close(gram(npy(trajs[s].chol)), gram(ref.chol), tol=1e-8, what=f"seq {s} cov")
+        assert abs(ell[s] - e) <= 1e-8 * max(1.0, abs(e))
+
+
+def test_graph_replay_matches_eager():
+    from parsmooth.batch import IteratedSmoother
+    T = 500
+    ys, _, (td, od) = _bearings_setup(T, seed=2)
+    m0 = np.array([[0.1, 0.2, 1.0, 0.0, 0.0]])
+    runner = IteratedSmoother([T], td, od, extended)
+    runner.set_data(t64(ys), t64(m0), t64(np.eye(5)[None]))
+    eager = [npy(x) for x in runner.run(4, use_graph=False)]
+    for _ in range(3):
+        graph = [npy(x) for x in runner.run(4, use_graph=True)]
+        torch.cuda.synchronize()
+        np.testing.assert_array_equal(graph[0], eager[0])
+        np.testing.assert_array_equal(graph[1], eager[1])
+
+
+# ---------------------------------------------------------------- FP32
+def test_fp32_bearings_vs_fp64_oracle():
+    """FP32 square-root IEKS, 5 iterations, T=1000: max relative error of the
+    smoothed mean against the FP64 oracle below 1e-4 of the trajectory scale."""
+    T = 1000
+    ys, (to, oo), (td64, od64) = _bearings_setup(T, seed=7)
+    m0 = np.array([0.1, 0.2, 1.0, 0.0, 0.0])
+    ref = O.iterated_smoothing(ys, O.MVNSqrt(m0, np.eye(5)), to, oo, O.extended, None, True,
+                               criterion=lambda i, *_: i < 5)
+    t32 = lambda a: torch.as_tensor(np.asarray(a), dtype=torch.float32, device=DEV)
+    td = FunctionalModel(td64.function, MVNSqrt(t32(npy(td64.mvn.mean)), t32(npy(td64.mvn.chol))))
+    od = FunctionalModel(od64.function, MVNSqrt(t32(npy(od64.mvn.mean)), t32(npy(od64.mvn.chol))))
+    got = parsmooth.iterated_smoothing(t32(ys), MVNSqrt(t32(m0), t32(np.eye(5))), td, od, extended,
+                                       criterion=lambda i, *_: i < 5)
+    assert got.mean.dtype == torch.float32
+    close(npy(got.mean), ref.mean, tol=1e-4, what="fp32 mean")
