@@ -139,6 +139,10 @@ int psm_smoother_combine(int32_t dtype, int32_t form, int32_t nx, int64_t n, con
  * ------------------------------------------------------------------------- */
 size_t psm_workspace_bytes(const psm_problem_t* prob);
 
+/* States per lane of the chunk passes this problem resolves to (prob->chunk,
+ * else the PSM_CHUNK environment variable, else a size-dependent default). */
+int psm_chunk_size(const psm_problem_t* prob);
+
 /* Parallel filter at a nominal trajectory.  Replaces
  * parsmooth/parallel/_filtering.py:13-61 (linearization at every step, element
  * construction, associative scan, x0 prepended, log-likelihood).
@@ -174,6 +178,14 @@ int psm_mean_sq_diff(int32_t dtype, int64_t n, const void* a, const void* b, dou
 
 /* Drop every cached hipGraph (call before freeing buffers they reference). */
 void psm_clear_graphs(void);
+
+/* Kernel timing for measurement: when enabled, each kernel launched by
+ * psm_filter / psm_smoother / psm_filter_smoother (outside graph capture) is
+ * bracketed by hipEvents on its stream.  psm_timing_report() waits for them
+ * and writes {"kernel": [total_ms, launches], ...} into buf (returns the full
+ * length), then forgets the records. */
+void psm_timing_enable(int32_t enable);
+int psm_timing_report(char* buf, size_t len);
 
 #ifdef __cplusplus
 }

@@ -125,6 +125,38 @@ static void push_model(std::vector<uint64_t>& k, const psm_model_t* m) {
   push(k, m->params); push(k, m->q_mean); push(k, m->q_chol);
 }
 
+// ---------------------------------------------------------------- timing
+struct TimingRec {
+  const char* name;
+  hipEvent_t a, b;
+};
+static std::mutex g_tmu;
+static bool g_timing = false;
+static std::vector<TimingRec> g_trecs;
+
+int timing_begin(const char* name, hipStream_t st) {
+  if (!g_timing) return -1;
+  hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
+  if (hipStreamIsCapturing(st, &cap) == hipSuccess && cap != hipStreamCaptureStatusNone) return -1;
+  TimingRec r{name, nullptr, nullptr};
+  if (hipEventCreate(&r.a) != hipSuccess || hipEventCreate(&r.b) != hipSuccess) return -1;
+  (void)hipEventRecord(r.a, st);
+  std::lock_guard<std::mutex> lock(g_tmu);
+  g_trecs.push_back(r);
+  return (int)g_trecs.size() - 1;
+}
+
+void timing_end(int slot, hipStream_t st) {
+  if (slot < 0) return;
+  hipEvent_t b;
+  {
+    std::lock_guard<std::mutex> lock(g_tmu);
+    if (slot >= (int)g_trecs.size()) return;
+    b = g_trecs[slot].b;
+  }
+  (void)hipEventRecord(b, st);
+}
+
 }  // namespace psm
 
 using namespace psm;
@@ -196,6 +228,11 @@ size_t psm_workspace_bytes(const psm_problem_t* p) {
   const OpsTable* t = need(p->dtype, p->form, p->nx, p->ny);
   if (!t) return 0;
   return t->workspace_bytes(p);
+}
+
+int psm_chunk_size(const psm_problem_t* p) {
+  if (check_problem(p)) return 0;
+  return make_plan(p, 8, 1, 1, 1).chunk;
 }
 
 int psm_filter(const psm_problem_t* p, const psm_model_t* trans, const psm_model_t* obs,
@@ -321,6 +358,47 @@ int psm_mean_sq_diff(int32_t dtype, int64_t n, const void* a, const void* b, dou
                        (const float*)b, out);
   else { set_error("bad dtype"); return PSM_E_ARG; }
   return hip_check("k_mean_sq_diff");
+}
+
+void psm_timing_enable(int32_t enable) {
+  std::lock_guard<std::mutex> lock(g_tmu);
+  g_timing = enable != 0;
+}
+
+int psm_timing_report(char* buf, size_t len) {
+  std::vector<TimingRec> recs;
+  {
+    std::lock_guard<std::mutex> lock(g_tmu);
+    recs.swap(g_trecs);
+  }
+  std::map<std::string, std::pair<double, long>> agg;
+  for (auto& r : recs) {
+    float ms = 0.f;
+    (void)hipEventSynchronize(r.b);
+    if (hipEventElapsedTime(&ms, r.a, r.b) == hipSuccess) {
+      auto& e = agg[r.name];
+      e.first += ms;
+      e.second += 1;
+    }
+    (void)hipEventDestroy(r.a);
+    (void)hipEventDestroy(r.b);
+  }
+  std::string out = "{";
+  bool first = true;
+  for (auto& kv : agg) {
+    char tmp[256];
+    snprintf(tmp, sizeof(tmp), "%s\"%s\": [%.6f, %ld]", first ? "" : ", ", kv.first.c_str(),
+             kv.second.first, kv.second.second);
+    out += tmp;
+    first = false;
+  }
+  out += "}";
+  if (buf && len > 0) {
+    size_t n = out.size() < len - 1 ? out.size() : len - 1;
+    memcpy(buf, out.data(), n);
+    buf[n] = 0;
+  }
+  return (int)out.size();
 }
 
 void psm_clear_graphs(void) {

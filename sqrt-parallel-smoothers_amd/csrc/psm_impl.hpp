@@ -21,6 +21,20 @@ inline int hip_check(const char* what) {
   return PSM_OK;
 }
 
+// ---------------------------------------------------------------- kernel timing
+// When enabled (psm_timing_enable), every launch of the whole-pass entry points
+// is bracketed by two hipEvents recorded on the launching stream; the report
+// aggregates elapsed times per kernel.
+int timing_begin(const char* name, hipStream_t st);   // -1 when disabled
+void timing_end(int slot, hipStream_t st);
+
+struct TimedScope {
+  int slot;
+  hipStream_t st;
+  TimedScope(const char* name, hipStream_t s) : slot(timing_begin(name, s)), st(s) {}
+  ~TimedScope() { timing_end(slot, st); }
+};
+
 // ---------------------------------------------------------------- workspace plan
 struct ScanLevel {
   int64_t n;      // items at this level
@@ -232,11 +246,14 @@ struct Impl {
 
   template <typename Agg>
   static int run_scan(const Plan& pl, char* ws, int reverse, hipStream_t st) {
+    const bool filt = (Agg::NQ == FA::NQ) && !reverse;
     for (int l = 0; l < pl.nlevels; ++l) {
       const ScanLevel& s = pl.lv[l];
       R* in = (R*)(ws + s.off_in);
       R* out = (R*)(ws + s.off_out);
       R* tot = (l + 1 < pl.nlevels) ? (R*)(ws + pl.lv[l + 1].off_in) : nullptr;
+      TimedScope ts_(l == 0 ? (filt ? "k_fscan_up_l0" : "k_sscan_up_l0")
+                            : (filt ? "k_fscan_up" : "k_sscan_up"), st);
       hipLaunchKernelGGL((k_scan_up<Agg, R, SQRT>), dim3((unsigned)s.nwg), dim3(kScanWG), 0, st,
                          (const R*)in, s.n, s.K, (l == 0) ? reverse : 0, out, tot, s.nwg);
       int rc = hip_check("k_scan_up");
@@ -244,6 +261,7 @@ struct Impl {
     }
     for (int l = pl.nlevels - 2; l >= 0; --l) {
       const ScanLevel& s = pl.lv[l];
+      TimedScope ts_(filt ? "k_fscan_down" : "k_sscan_down", st);
       hipLaunchKernelGGL((k_scan_down<Agg, R, SQRT>), dim3((unsigned)s.nwg), dim3(kScanWG), 0, st,
                          (R*)(ws + s.off_out), s.n, s.K, (l == 0) ? reverse : 0,
                          (const R*)(ws + pl.lv[l + 1].off_out), pl.lv[l + 1].n);
@@ -259,6 +277,7 @@ struct Impl {
     int use = 0;
     SigmaView<R> sv = sigma_view<R>(sg, &use);
     ModelView<R> obv = ob ? model_view<R>(ob) : model_view<R>(tr);
+    TimedScope ts_("k_linearize_states", st);
     hipLaunchKernelGGL((k_linearize_states<R, NX, NY, SQRT>), dim3(grid_for(pl.M, kLaneWG)),
                        dim3(kLaneWG), 0, st, pl.M, model_view<R>(tr), obv, ob ? 1 : 0, sv, use,
                        (const R*)nm, (const R*)nL, (R*)(ws + pl.off_lin));
@@ -282,17 +301,23 @@ struct Impl {
       if (rc) return rc;
     }
     const R* lin = (const R*)(ws + pl.off_lin);
+    {
+    TimedScope ts_("k_filter_reduce", st);
     hipLaunchKernelGGL((k_filter_reduce<R, NX, NY, SQRT>), dim3(grid_for(pl.nchunks, kLaneWG)),
                        dim3(kLaneWG), 0, st, pl.M, pl.chunk, pl.nchunks, p->state_offsets,
                        p->n_seq, lin, (const R*)ys, (const R*)m0, (const R*)L0,
                        (R*)(ws + pl.lv[0].off_in));
+    }
     if ((rc = hip_check("k_filter_reduce"))) return rc;
     if ((rc = run_scan<FA>(pl, ws, 0, st))) return rc;
     double* ell_state = ell ? (double*)(ws + pl.off_ell) : nullptr;
+    {
+    TimedScope ts_("k_filter_apply", st);
     hipLaunchKernelGGL((k_filter_apply<R, NX, NY, SQRT>), dim3(grid_for(pl.nchunks, kLaneWG)),
                        dim3(kLaneWG), 0, st, pl.M, pl.chunk, pl.nchunks, p->state_offsets,
                        p->n_seq, lin, (const R*)ys, (const R*)m0, (const R*)L0,
                        (const R*)(ws + pl.lv[0].off_out), (R*)om, (R*)oL, ell_state);
+    }
     if ((rc = hip_check("k_filter_apply"))) return rc;
     if (ell) {
       hipLaunchKernelGGL(k_ell_reduce<256>, dim3((unsigned)p->n_seq), dim3(256), 0, st,
@@ -317,16 +342,22 @@ struct Impl {
       rc = linearize_states(pl, tr, nullptr, sg, nm, nL, ws, st);
       if (rc) return rc;
     }
+    {
+    TimedScope ts_("k_smoother_reduce", st);
     hipLaunchKernelGGL((k_smoother_reduce<R, NX, NY, SQRT>), dim3(grid_for(pl.nchunks, kLaneWG)),
                        dim3(kLaneWG), 0, st, pl.M, pl.chunk, pl.nchunks, p->state_offsets,
                        p->n_seq, (const R*)(ws + pl.off_lin), (const R*)fm, (const R*)fL,
                        (R*)(ws + pl.off_selem), (R*)(ws + pl.lv[0].off_in));
+    }
     if ((rc = hip_check("k_smoother_reduce"))) return rc;
     if ((rc = run_scan<SA>(pl, ws, 1, st))) return rc;
-    hipLaunchKernelGGL((k_smoother_apply<R, NX, SQRT>), dim3(grid_for(pl.nchunks, kLaneWG)),
-                       dim3(kLaneWG), 0, st, pl.M, pl.chunk, pl.nchunks,
-                       (const R*)(ws + pl.off_selem), (const R*)(ws + pl.lv[0].off_out), (R*)om,
-                       (R*)oL);
+    {
+      TimedScope ts_("k_smoother_apply", st);
+      hipLaunchKernelGGL((k_smoother_apply<R, NX, SQRT>), dim3(grid_for(pl.nchunks, kLaneWG)),
+                         dim3(kLaneWG), 0, st, pl.M, pl.chunk, pl.nchunks,
+                         (const R*)(ws + pl.off_selem), (const R*)(ws + pl.lv[0].off_out), (R*)om,
+                         (R*)oL);
+    }
     return hip_check("k_smoother_apply");
   }
 

@@ -32,6 +32,7 @@ EXPORTS = (
     "psm_version", "psm_last_error", "psm_supported", "psm_linearize", "psm_filter_elements",
     "psm_filter_combine", "psm_smoother_elements", "psm_smoother_combine", "psm_workspace_bytes",
     "psm_filter", "psm_smoother", "psm_filter_smoother", "psm_mean_sq_diff", "psm_clear_graphs",
+    "psm_timing_enable", "psm_timing_report", "psm_chunk_size",
 )
 
 
@@ -79,6 +80,9 @@ _SIGS = {
                             + [_P] * 9 + [_P, ctypes.c_size_t, _I32, _P]),
     "psm_mean_sq_diff": (ctypes.c_int, [_I32, _I64, _P, _P, _P, _P]),
     "psm_clear_graphs": (None, []),
+    "psm_timing_enable": (None, [_I32]),
+    "psm_chunk_size": (ctypes.c_int, [ctypes.POINTER(PsmProblem)]),
+    "psm_timing_report": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_size_t]),
 }
 
 
@@ -137,3 +141,15 @@ def dtype_code(dtype: torch.dtype) -> int:
 def ptr_array(tensors):
     arr = (ctypes.c_void_p * len(tensors))(*[t.data_ptr() for t in tensors])
     return arr
+
+
+def timing_enable(on: bool = True):
+    lib().psm_timing_enable(1 if on else 0)
+
+
+def timing_report() -> dict:
+    """Per-kernel {name: (total_ms, launches)} since the last report."""
+    import json
+    buf = ctypes.create_string_buffer(1 << 16)
+    lib().psm_timing_report(buf, len(buf))
+    return {k: tuple(v) for k, v in json.loads(buf.value.decode()).items()}

@@ -50,6 +50,11 @@ class IteratedSmoother:
         self.m0 = None
         self.L0 = None
 
+    def chunk_size(self) -> int:
+        """States per lane in the chunk passes (as resolved by the library)."""
+        import ctypes
+        return int(N.lib().psm_chunk_size(ctypes.byref(self.prob.struct)))
+
     @property
     def state_offsets(self) -> np.ndarray:
         return self.prob.state_offsets_host
