@@ -90,20 +90,36 @@ def time_runner(runner, init, n_iter, steps, warmup, use_graph, kernel_timing=Fa
     return dt, out, rep
 
 
-def cpu_baseline(T_sample, ys, n_iter):
-    """FP64 CPU restatement (numpy oracle, parallel formulation) on a bounded sample."""
+def cpu_baseline(T_sample, ys, n_iter, repeats=1):
+    """FP64 C restatement of the algorithm (sequential square-root filter and
+    smoother at each nominal, one core) on the same inputs; falls back to the
+    numpy restatement (parallel formulation) when the C library is not built."""
     from oracle import psm_oracle as O
     Q, R = O.bearings_parameters(0.01, 0.1, 0.5, 0.01)
-    to = O.FunctionalModel(O.CoordinatedTurn(0.01), O.MVNSqrt(np.zeros(5), np.linalg.cholesky(Q)))
-    oo = O.FunctionalModel(O.Bearings((-1.5, 0.5), (1.0, 1.0)), O.MVNSqrt(np.zeros(2), np.linalg.cholesky(R)))
+    cQ, cR = np.linalg.cholesky(Q), np.linalg.cholesky(R)
     y = ys[:T_sample]
-    init = O.MVNSqrt(np.tile(np.array([-1.0, -1.0, 6.0, 4.0, 2.0]), (T_sample + 1, 1)),
-                     np.tile(np.eye(5), (T_sample + 1, 1, 1)))
-    t0 = time.perf_counter()
-    O.iterated_smoothing(y, O.MVNSqrt(np.array([0.1, 0.2, 1.0, 0.0, 0.0]), np.eye(5)), to, oo, O.extended,
-                         init, True, criterion=lambda i, *_: i < n_iter)
-    dt = time.perf_counter() - t0
-    return T_sample * n_iter / dt, dt
+    m0 = np.array([0.1, 0.2, 1.0, 0.0, 0.0])
+    init_m = np.tile(np.array([-1.0, -1.0, 6.0, 4.0, 2.0]), (T_sample + 1, 1))
+    try:
+        from oracle import psm_oracle_c as C
+        C.lib()
+        t0 = time.perf_counter()
+        for _ in range(repeats):
+            C.iterated_smoother(y, m0, np.eye(5), init_m, ("ct", 0.01), ("bearings", ((-1.5, 0.5), (1.0, 1.0))),
+                                np.zeros(5), cQ, np.zeros(2), cR, n_iter)
+        dt = time.perf_counter() - t0
+        desc = f"C FP64 restatement (sequential sqrt filter/smoother, 1 thread), T={T_sample}, {n_iter} iterations x {repeats}"
+    except RuntimeError:
+        to = O.FunctionalModel(O.CoordinatedTurn(0.01), O.MVNSqrt(np.zeros(5), cQ))
+        oo = O.FunctionalModel(O.Bearings((-1.5, 0.5), (1.0, 1.0)), O.MVNSqrt(np.zeros(2), cR))
+        init = O.MVNSqrt(init_m, np.tile(np.eye(5), (T_sample + 1, 1, 1)))
+        t0 = time.perf_counter()
+        for _ in range(repeats):
+            O.iterated_smoothing(y, O.MVNSqrt(m0, np.eye(5)), to, oo, O.extended, init, True,
+                                 criterion=lambda i, *_: i < n_iter)
+        dt = time.perf_counter() - t0
+        desc = f"numpy FP64 restatement (parallel formulation), T={T_sample}, {n_iter} iterations x {repeats}"
+    return T_sample * n_iter * repeats / dt, dt, desc
 
 
 def main():
@@ -115,7 +131,10 @@ def main():
     ap.add_argument("--n-iter", type=int, default=10)
     ap.add_argument("--graph", type=int, default=1)
     ap.add_argument("--cpu-sample", type=int, default=131072)
+    ap.add_argument("--cpu-repeats", type=int, default=4)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="run one 256 MiB device copy first (byte-counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (got --gpus {args.gpus}); multi-GPU runs are not part "
@@ -123,6 +142,16 @@ def main():
         sys.exit(2)
     import torch
     torch.cuda.set_device(0)
+    if args.copy_calibration:
+        # a plain device copy of known size, for scaling HBM byte counters
+        nbytes = 256 << 20
+        src = torch.empty(nbytes // 8, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        src.fill_(1.0)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
     T, n_iter = args.T, args.n_iter
     ys = make_inputs(T)
 
@@ -162,10 +191,9 @@ def main():
             "note": "FP64 VALU work (no MFMA on <=10x10 blocks); peak = MI355X FP64 vector rate"}
     cpu = None
     if not args.no_cpu:
-        v, dtc = cpu_baseline(min(args.cpu_sample, T), ys, n_iter)
+        v, dtc, desc = cpu_baseline(min(args.cpu_sample, T), ys, n_iter, args.cpu_repeats)
         cpu = {"value": v, "unit": "step-it/s", "cores": 1, "kind": "port",
-               "sample": f"numpy FP64 oracle, parallel formulation, T={min(args.cpu_sample, T)}, "
-                         f"{n_iter} iterations, {dtc:.1f} s"}
+               "sample": f"{desc}, {dtc:.1f} s"}
     out = {
         "metric": "smoothed steps/s (FP64 sqrt IEKS, bearings-only CT, 10 iterations)",
         "value": value, "unit": "step-it/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,

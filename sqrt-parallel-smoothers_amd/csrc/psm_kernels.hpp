@@ -34,6 +34,7 @@
 #include <vector>
 
 #include "psm_core.hpp"
+#include "psm_scan_mem.hpp"
 
 namespace psm {
 
@@ -164,61 +165,56 @@ __global__ __launch_bounds__(kLaneWG) void k_filter_reduce(
 // ============================================================================
 // 3. exclusive scan of aggregates (filter: forward, smoother: reverse)
 // ============================================================================
-template <typename Agg, typename R>
-PSM_DEV void lds_store(R* lds, int t, const Agg& a) {
-  store_agg(lds, (int64_t)kScanWG, (int64_t)t, a);
-}
-template <typename Agg, typename R>
-PSM_DEV void lds_load(const R* lds, int t, Agg& a) {
-  load_agg(lds, (int64_t)kScanWG, (int64_t)t, a);
-}
-
-// in/out: n items (SoA, stride n). Item index of scan position p is
-// (reverse ? n - 1 - p : p).  Workgroup w covers positions [w*WG*K, (w+1)*WG*K).
-// out_excl[item] = exclusive prefix within the workgroup; totals[w] = the
-// workgroup's inclusive total (scan order).
+// in/out: n items (SoA, stride n).  Item index of scan position p is
+// (reverse ? n - 1 - p : p).  Workgroup w covers positions [w*WG*K, (w+1)*WG*K):
+// every lane folds its K items, a Kogge-Stone scan over the lanes runs in a
+// double-buffered LDS array, and the lane then writes the exclusive prefix of
+// each of its items (within the workgroup) to out_excl; totals[w] receives the
+// workgroup's inclusive total.  Aggregates live in LDS / HBM, never whole in
+// registers: the operator reads its operands from memory as it needs them.
 template <typename Agg, typename R, bool SQRT>
 __global__ __launch_bounds__(kScanWG) void k_scan_up(const R* __restrict__ in, int64_t n, int K,
                                                      int reverse, R* __restrict__ out_excl,
                                                      R* __restrict__ totals, int64_t ntot) {
-  __shared__ R lds[Agg::NQ * kScanWG];
+  using V = typename ViewOf<Agg>::type;
+  constexpr int NX = Agg_nx<Agg>::value;
+  constexpr int NQ = Agg::NQ;
+  __shared__ R lds[2 * NQ * kScanWG];
   const int t = threadIdx.x;
+  auto slot = [&](int buf, int lane) { return V{lds + buf * NQ * kScanWG, (int64_t)kScanWG, (int64_t)lane}; };
+  R* inm = const_cast<R*>(in);
   const int64_t base = ((int64_t)blockIdx.x * kScanWG + t) * K;
-  Agg acc;
-  set_identity(acc);
+  int cur = 0;
+  view_identity(slot(0, t));
   for (int j = 0; j < K; ++j) {
     const int64_t p = base + j;
-    if (p < n) {
-      Agg e;
-      load_agg(in, n, reverse ? n - 1 - p : p, e);
-      combine_into<R, Agg_nx<Agg>::value, SQRT>(acc, e);
-      copy_agg(e, acc);
-    }
+    if (p >= n) break;
+    const V e{inm, n, reverse ? n - 1 - p : p};
+    combine_mem<R, NX, SQRT>(slot(cur, t), e, slot(1 - cur, t));
+    cur = 1 - cur;
   }
+  if (cur == 1) view_copy(slot(1, t), slot(0, t));
+  cur = 0;
   for (int d = 1; d < kScanWG; d <<= 1) {
-    lds_store(lds, t, acc);
     __syncthreads();
-    Agg other;
-    if (t >= d) lds_load(lds, t - d, other);
-    __syncthreads();
-    if (t >= d) combine_into<R, Agg_nx<Agg>::value, SQRT>(other, acc);
+    if (t >= d) combine_mem<R, NX, SQRT>(slot(cur, t - d), slot(cur, t), slot(1 - cur, t));
+    else view_copy(slot(cur, t), slot(1 - cur, t));
+    cur = 1 - cur;
   }
-  lds_store(lds, t, acc);
   __syncthreads();
-  if (t == kScanWG - 1 && totals != nullptr) store_agg(totals, ntot, (int64_t)blockIdx.x, acc);
-  Agg run;
-  if (t == 0) set_identity(run);
-  else lds_load(lds, t - 1, run);
+  if (t == kScanWG - 1 && totals != nullptr)
+    view_copy(slot(cur, t), V{totals, ntot, (int64_t)blockIdx.x});
+  // running exclusive prefix in this lane's slot of the other buffer
+  const V run = slot(1 - cur, t);
+  if (t == 0) view_identity(run);
+  else view_copy(slot(cur, t - 1), run);
   for (int j = 0; j < K; ++j) {
     const int64_t p = base + j;
-    if (p < n) {
-      const int64_t item = reverse ? n - 1 - p : p;
-      store_agg(out_excl, n, item, run);
-      Agg e;
-      load_agg(in, n, item, e);
-      combine_into<R, Agg_nx<Agg>::value, SQRT>(run, e);
-      copy_agg(e, run);
-    }
+    if (p >= n) break;
+    const int64_t item = reverse ? n - 1 - p : p;
+    const V o{out_excl, n, item};
+    view_copy(run, o);
+    combine_mem<R, NX, SQRT>(o, V{inm, n, item}, run);
   }
 }
 
@@ -227,20 +223,17 @@ template <typename Agg, typename R, bool SQRT>
 __global__ __launch_bounds__(kScanWG) void k_scan_down(R* __restrict__ out_excl, int64_t n, int K,
                                                        int reverse, const R* __restrict__ upper,
                                                        int64_t nupper) {
+  using V = typename ViewOf<Agg>::type;
+  constexpr int NX = Agg_nx<Agg>::value;
   if (blockIdx.x == 0) return;
   const int t = threadIdx.x;
   const int64_t base = ((int64_t)blockIdx.x * kScanWG + t) * K;
-  Agg up;
-  load_agg(upper, nupper, (int64_t)blockIdx.x, up);
+  const V up{const_cast<R*>(upper), nupper, (int64_t)blockIdx.x};
   for (int j = 0; j < K; ++j) {
     const int64_t p = base + j;
-    if (p < n) {
-      const int64_t item = reverse ? n - 1 - p : p;
-      Agg e;
-      load_agg(out_excl, n, item, e);
-      combine_into<R, Agg_nx<Agg>::value, SQRT>(up, e);
-      store_agg(out_excl, n, item, e);
-    }
+    if (p >= n) break;
+    const V e{out_excl, n, reverse ? n - 1 - p : p};
+    combine_mem<R, NX, SQRT>(up, e, e);
   }
 }
 

@@ -333,3 +333,46 @@ def test_fixed_point_scalar_known_answer():
     ga, gb = O.fixed_point_vjp(lambda u: a * u, lambda u: (u * x_star, u), 1.0, n)
     assert ga == pytest.approx(x_star / (1 - a), rel=1e-7)
     assert gb == pytest.approx(1 / (1 - a), rel=1e-7)
+
+
+# ---------------------------------------------------------------- C restatement (CPU baseline)
+def _c_oracle():
+    import os
+    import subprocess
+    from oracle import psm_oracle_c as C
+    so = os.path.join(os.path.dirname(C.__file__), "liboracle_c.so")
+    if not os.path.exists(so):
+        subprocess.run(["make", "-C", os.path.dirname(so)], check=True, capture_output=True)
+    return C
+
+
+def test_c_restatement_matches_numpy_oracle_bearings():
+    C = _c_oracle()
+    from parsmooth.data import bearings_data
+    T = 200
+    _, _, ys = bearings_data(np.array([0.1, 0.2, 1.0, 0.0]), 0.01, 0.5, T, (-1.5, 0.5), (1.0, 1.0), random_state=4)
+    Q, R = O.bearings_parameters(0.01, 0.1, 0.5, 0.01)
+    cQ, cR = np.linalg.cholesky(Q), np.linalg.cholesky(R)
+    m0 = np.array([0.1, 0.2, 1.0, 0.0, 0.0])
+    init = np.tile([-1.0, -1.0, 6.0, 4.0, 2.0], (T + 1, 1))
+    to = O.FunctionalModel(O.CoordinatedTurn(0.01), O.MVNSqrt(np.zeros(5), cQ))
+    oo = O.FunctionalModel(O.Bearings((-1.5, 0.5), (1.0, 1.0)), O.MVNSqrt(np.zeros(2), cR))
+    ref, ell = O.iterated_smoothing(ys, O.MVNSqrt(m0, np.eye(5)), to, oo, O.extended,
+                                    O.MVNSqrt(init, np.tile(np.eye(5), (T + 1, 1, 1))), True,
+                                    criterion=lambda i, *_: i < 3, return_loglikelihood=True)
+    m, L, e = C.iterated_smoother(ys, m0, np.eye(5), init, ("ct", 0.01), ("bearings", ((-1.5, 0.5), (1.0, 1.0))),
+                                  np.zeros(5), cQ, np.zeros(2), cR, 3, True)
+    np.testing.assert_allclose(m, ref.mean, atol=1e-10)
+    np.testing.assert_allclose(L, ref.chol, atol=1e-10)
+    assert e == pytest.approx(ell, rel=1e-12)
+
+
+def test_c_restatement_linear_closed_form():
+    C = _c_oracle()
+    x0, cx0, (F, Q, b, H, R, c), ys, (ts, os_), _ = _lin_problem(3, 2, 0, 6)
+    (fm, fP), (sm, sP), ell = O.joint_gaussian_posterior(ys, x0.mean, x0.cov, F, Q, b, H, R, c)
+    m, L, e = C.iterated_smoother(ys, cx0.mean, cx0.chol, np.zeros((7, 3)), ("linear", F), ("linear", H),
+                                  b, np.linalg.cholesky(Q), c, np.linalg.cholesky(R), 1, True)
+    np.testing.assert_allclose(m, sm, atol=1e-8)
+    np.testing.assert_allclose(L @ np.swapaxes(L, 1, 2), sP, atol=1e-8)
+    assert e == pytest.approx(ell, rel=1e-9)
