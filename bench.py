@@ -69,6 +69,30 @@ def build_runner(T, ys, dtype, n_seq=1):
     return runner, init
 
 
+def build_batch_runner(n_seq, lo, hi, seed, dtype, method):
+    """Ragged batch: lengths U{lo..hi}, per-sequence prior mean ~ N([0.1,0.2,1,0,0], 0.1 I),
+    chol I; initial nominal = the prior mean at every state, identity chol."""
+    import torch
+    from parsmooth import FunctionalModel, MVNSqrt
+    from parsmooth.batch import IteratedSmoother
+    from parsmooth.data import bearings_data_batch
+    from parsmooth.models import make_parameters
+    rng = np.random.default_rng(seed)
+    lengths = rng.integers(lo, hi + 1, n_seq)
+    m0 = np.array([0.1, 0.2, 1.0, 0.0, 0.0]) + np.sqrt(0.1) * rng.standard_normal((n_seq, 5))
+    _, obs = bearings_data_batch(lengths, m0[:, :4], 0.01, 0.5, (-1.5, 0.5), (1.0, 1.0), seed=seed + 1)
+    dev = torch.device("cuda", 0)
+    Q, R, obs_f, trans_f = make_parameters(0.01, 0.1, 0.5, 0.01, (-1.5, 0.5), (1.0, 1.0))
+    t = lambda a: torch.as_tensor(np.asarray(a, dtype=np.float64)).to(dev, dtype)
+    trans = FunctionalModel(trans_f, MVNSqrt(t(np.zeros(5)), t(np.linalg.cholesky(Q))))
+    obsm = FunctionalModel(obs_f, MVNSqrt(t(np.zeros(2)), t(np.linalg.cholesky(R))))
+    runner = IteratedSmoother(lengths, trans, obsm, method, dtype=dtype)
+    runner.set_data(t(np.concatenate(obs)), t(m0), t(np.tile(np.eye(5), (n_seq, 1, 1))))
+    reps = lengths + 1
+    init = (t(np.repeat(m0, reps, axis=0)), t(np.tile(np.eye(5), (int(reps.sum()), 1, 1))))
+    return runner, init, int(lengths.sum())
+
+
 def time_runner(runner, init, n_iter, steps, warmup, use_graph, kernel_timing=False):
     import torch
     from parsmooth import _native as N
@@ -133,6 +157,7 @@ def main():
     ap.add_argument("--cpu-sample", type=int, default=131072)
     ap.add_argument("--cpu-repeats", type=int, default=4)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--batch-seqs", type=int, default=4096)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run one 256 MiB device copy first (byte-counter calibration)")
     args = ap.parse_args()
@@ -163,7 +188,9 @@ def main():
 
     # FP32
     r32, init32 = build_runner(T, ys, torch.float32)
+    r32.prob.fallback_counter.zero_()
     dt32, out32, _ = time_runner(r32, init32, n_iter, args.steps, args.warmup, bool(args.graph))
+    fb32 = r32.fallback_count() / max(1, args.steps + args.warmup)
     mean32 = out32[0].double().cpu().numpy()
     sc = max(1.0, np.abs(mean64).max())
     rel32_end = float(np.abs(mean32 - mean64).max() / sc)
@@ -176,6 +203,19 @@ def main():
     torch.cuda.synchronize()
     d1 = np.abs(p32[0].double().cpu().numpy() - p64[0].cpu().numpy()).max(1) / sc
     rel32_pass = {"max": float(d1.max()), "median": float(np.median(d1)), "p99": float(np.quantile(d1, 0.99))}
+
+    batch = None
+    if args.batch_seqs > 0:
+        from parsmooth.linearization import cubature
+        rb, ib, nsteps = build_batch_runner(args.batch_seqs, 256, 8192, 0, torch.float32, cubature)
+        dtb, _, repb = time_runner(rb, ib, 5, max(1, args.steps // 2), 1, bool(args.graph))
+        _, _, repb = time_runner(rb, ib, 5, 1, 0, False, kernel_timing=True)
+        batch = {"workload": f"{args.batch_seqs} bearings-only sequences, lengths U[256, 8192], FP32 sqrt "
+                             "cubature IPLS, 5 iterations, one launch sequence per pass",
+                 "steps": nsteps, "ms_per_step": dtb * 1e3, "value": nsteps * 5 / dtb,
+                 "unit": "step-it/s", "kernel_ms": {k: v[0] / v[1] for k, v in repb.items()}}
+        del rb, ib
+        torch.cuda.empty_cache()
 
     value = T * n_iter / dt64
     # dominant kernel by total time
@@ -203,8 +243,10 @@ def main():
                    "T": T, "nx": NX, "ny": NY, "n_seq": 1, "n_iter": n_iter, "graph": bool(args.graph)},
         "fp32_value": T * n_iter / dt32, "fp32_ms_per_step": dt32 * 1e3, "fp32_over_fp64": dt64 / dt32,
         "fp32_rel_err_single_pass": rel32_pass, "fp32_rel_err_after_iterations": rel32_end,
+        "fp32_fp64_fallbacks_per_step": fb32,
         "alg_bytes_per_step_it": ALG_BYTES_WORDS * 8, "alg_flops_per_step_it": ALG_FLOPS,
         "kernel_ms": {k: v[0] / v[1] for k, v in rep.items()},
+        "batch_fp32_cubature": batch,
         "roofline": roof,
         "cpu_baseline": cpu,
     }

@@ -83,6 +83,15 @@ typedef struct psm_problem {
   int32_t chunk;         /* states per lane in the chunk passes (0: default) */
   int64_t n_states;      /* total states = sum(T_s + 1) */
   const int64_t* state_offsets;  /* device, n_seq + 1 */
+  /* FP32 square-root form only: when a triangular factor produced by a Kalman
+   * step, an absorbed step, a smoother element / application or a scan
+   * combination has min|diag| / max|diag| below fallback_ratio, that operation
+   * is recomputed in FP64 from the same FP32 inputs and rounded back.  0
+   * disables the check.  fallback_count (device, may be NULL) is incremented
+   * once per recomputation; the caller zeroes it. */
+  float fallback_ratio;
+  int32_t pad_;
+  unsigned long long* fallback_count;
 } psm_problem_t;
 
 int psm_version(void);

@@ -287,6 +287,12 @@ int psm_filter_smoother(const psm_problem_t* p, const psm_model_t* trans, const 
   key.push_back((uint64_t)p->n_seq);
   key.push_back((uint64_t)p->n_states);
   push(key, p->state_offsets);
+  {
+    uint32_t fr;
+    memcpy(&fr, &p->fallback_ratio, sizeof(fr));
+    key.push_back((uint64_t)fr);
+  }
+  push(key, p->fallback_count);
   push_model(key, trans);
   push_model(key, obs);
   if (sigma && sigma->npts > 0) {

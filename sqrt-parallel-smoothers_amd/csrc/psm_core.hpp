@@ -14,6 +14,8 @@
 // (parsmooth/parallel/_smoothing.py:72-85, _operators.py:104-125); the last
 // state of a sequence is the HEAD (E = 0).
 #pragma once
+#include <type_traits>
+
 #include "psm_models.hpp"
 
 namespace psm {
@@ -1095,4 +1097,183 @@ PSM_DEV void filter_element(const Step<R, NX, NY>& s, const R (&m)[NX], const R 
   else std_filter_element<R, NX, NY>(s, m, L, y, out);
 }
 
+// ============================================================================
+// FP32 square-root form: FP64 re-triangularisation fallback
+// ============================================================================
+struct Fallback {
+  float thr;                    // min|diag| / max|diag| threshold (0: off)
+  unsigned long long* cnt;      // incremented per FP64 recomputation (may be null)
+};
+
+template <typename R, bool SQRT>
+constexpr bool kFallback = SQRT && std::is_same<R, float>::value;
+
+// true when min|diag| < thr * max|diag| (or a diagonal entry is not finite)
+template <typename R, int N>
+PSM_DEV bool ill_conditioned(const R (&L)[N][N], float thr) {
+  R mn = abs_(L[0][0]), mx = mn;
+#pragma unroll
+  for (int i = 1; i < N; ++i) {
+    const R d = abs_(L[i][i]);
+    mn = d < mn ? d : mn;
+    mx = d > mx ? d : mx;
+  }
+  return !(mn >= R(thr) * mx);
+}
+
+PSM_DEV void count_fallback(const Fallback& fb) {
+  if (fb.cnt) atomicAdd(fb.cnt, 1ull);
+}
+
+template <int N> PSM_DEV void cvt(const float (&a)[N], double (&b)[N]) {
+#pragma unroll
+  for (int i = 0; i < N; ++i) b[i] = a[i];
+}
+template <int N> PSM_DEV void cvt(const double (&a)[N], float (&b)[N]) {
+#pragma unroll
+  for (int i = 0; i < N; ++i) b[i] = (float)a[i];
+}
+template <int M, int N> PSM_DEV void cvt(const float (&a)[M][N], double (&b)[M][N]) {
+#pragma unroll
+  for (int i = 0; i < M; ++i)
+#pragma unroll
+    for (int j = 0; j < N; ++j) b[i][j] = a[i][j];
+}
+template <int M, int N> PSM_DEV void cvt(const double (&a)[M][N], float (&b)[M][N]) {
+#pragma unroll
+  for (int i = 0; i < M; ++i)
+#pragma unroll
+    for (int j = 0; j < N; ++j) b[i][j] = (float)a[i][j];
+}
+template <int NX, int NY>
+PSM_DEV void cvt_step(const Step<float, NX, NY>& a, Step<double, NX, NY>& b) {
+  cvt(a.F, b.F); cvt(a.cq, b.cq); cvt(a.bt, b.bt); cvt(a.H, b.H); cvt(a.cr, b.cr); cvt(a.c, b.c);
+}
+template <int NX>
+PSM_DEV void cvt_agg(const FAgg<float, NX>& a, FAgg<double, NX>& b) {
+  cvt(a.A, b.A); cvt(a.b, b.b); cvt(a.U, b.U); cvt(a.eta, b.eta); cvt(a.Z, b.Z); b.flag = a.flag;
+}
+template <int NX>
+PSM_DEV void cvt_agg(const FAgg<double, NX>& a, FAgg<float, NX>& b) {
+  cvt(a.A, b.A); cvt(a.b, b.b); cvt(a.U, b.U); cvt(a.eta, b.eta); cvt(a.Z, b.Z); b.flag = a.flag;
+}
+template <int NX>
+PSM_DEV void cvt_agg(const SAgg<float, NX>& a, SAgg<double, NX>& b) {
+  cvt(a.g, b.g); cvt(a.E, b.E); cvt(a.D, b.D); b.flag = a.flag;
+}
+template <int NX>
+PSM_DEV void cvt_agg(const SAgg<double, NX>& a, SAgg<float, NX>& b) {
+  cvt(a.g, b.g); cvt(a.E, b.E); cvt(a.D, b.D); b.flag = a.flag;
+}
+
+template <typename R, int NX, int NY, bool SQRT>
+PSM_DEV double kf_step_fb(const Step<R, NX, NY>& s, const R (&y)[NY], R (&m)[NX], R (&L)[NX][NX],
+                          const Fallback& fb) {
+  if constexpr (kFallback<R, SQRT>) {
+    float m0[NX], L0[NX][NX];
+    copy(m, m0);
+    copy(L, L0);
+    double e = sqrt_kf_step<float, NX, NY>(s, y, m, L);
+    if (fb.thr > 0.f && ill_conditioned(L, fb.thr)) {
+      Step<double, NX, NY> sd;
+      cvt_step(s, sd);
+      double yd[NY], md[NX], Ld[NX][NX];
+      cvt(y, yd); cvt(m0, md); cvt(L0, Ld);
+      e = sqrt_kf_step<double, NX, NY>(sd, yd, md, Ld);
+      cvt(md, m);
+      cvt(Ld, L);
+      count_fallback(fb);
+    }
+    return e;
+  } else {
+    return kf_step<R, NX, NY, SQRT>(s, y, m, L);
+  }
+}
+
+template <typename R, int NX, int NY, bool SQRT>
+PSM_DEV void absorb_fb(FAgg<R, NX>& a, const Step<R, NX, NY>& s, const R (&y)[NY], const Fallback& fb) {
+  if constexpr (kFallback<R, SQRT>) {
+    FAgg<float, NX> a0;
+    copy_agg(a, a0);
+    absorb<float, NX, NY, true>(a, s, y);
+    if (fb.thr > 0.f && ill_conditioned(a.U, fb.thr)) {
+      FAgg<double, NX> ad;
+      cvt_agg(a0, ad);
+      Step<double, NX, NY> sd;
+      cvt_step(s, sd);
+      double yd[NY];
+      cvt(y, yd);
+      absorb<double, NX, NY, true>(ad, sd, yd);
+      cvt_agg(ad, a);
+      count_fallback(fb);
+    }
+  } else {
+    absorb<R, NX, NY, SQRT>(a, s, y);
+  }
+}
+
+template <typename R, int NX, bool SQRT>
+PSM_DEV void smoother_element_fb(const R (&F)[NX][NX], const R (&cq)[NX][NX], const R (&bt)[NX],
+                                 const R (&m)[NX], const R (&L)[NX][NX], SAgg<R, NX>& out,
+                                 const Fallback& fb) {
+  smoother_element<R, NX, SQRT>(F, cq, bt, m, L, out);
+  if constexpr (kFallback<R, SQRT>) {
+    if (fb.thr > 0.f && ill_conditioned(out.D, fb.thr)) {
+      double Fd[NX][NX], cqd[NX][NX], btd[NX], md[NX], Ld[NX][NX];
+      cvt(F, Fd); cvt(cq, cqd); cvt(bt, btd); cvt(m, md); cvt(L, Ld);
+      SAgg<double, NX> od;
+      sqrt_smoother_element<double, NX>(Fd, cqd, btd, md, Ld, od);
+      cvt_agg(od, out);
+      count_fallback(fb);
+    }
+  }
+}
+
+template <typename R, int NX, bool SQRT>
+PSM_DEV void smoother_apply_fb(const SAgg<R, NX>& e, R (&m)[NX], R (&Ds)[NX][NX], const Fallback& fb) {
+  if constexpr (kFallback<R, SQRT>) {
+    float m0[NX], D0[NX][NX];
+    copy(m, m0);
+    copy(Ds, D0);
+    sqrt_smoother_apply<float, NX>(e, m, Ds);
+    if (fb.thr > 0.f && ill_conditioned(Ds, fb.thr)) {
+      SAgg<double, NX> ed;
+      cvt_agg(e, ed);
+      double md[NX], Dd[NX][NX];
+      cvt(m0, md); cvt(D0, Dd);
+      sqrt_smoother_apply<double, NX>(ed, md, Dd);
+      cvt(md, m);
+      cvt(Dd, Ds);
+      count_fallback(fb);
+    }
+  } else {
+    smoother_apply<R, NX, SQRT>(e, m, Ds);
+  }
+}
+
+// e2 <- e1 (+) e2 for smoother aggregates held in registers, with the fallback
+template <typename R, int NX, bool SQRT>
+PSM_DEV void combine_into_fb(const SAgg<R, NX>& e1, SAgg<R, NX>& e2, const Fallback& fb) {
+  if constexpr (kFallback<R, SQRT>) {
+    if (e2.flag == FLAG_HEAD || e1.flag == FLAG_IDENT || e2.flag == FLAG_IDENT) {
+      combine_into<R, NX, SQRT>(e1, e2);
+      return;
+    }
+    SAgg<float, NX> b0;
+    copy_agg(e2, b0);
+    combine_into<R, NX, SQRT>(e1, e2);
+    if (fb.thr > 0.f && ill_conditioned(e2.D, fb.thr)) {
+      SAgg<double, NX> ad, bd;
+      cvt_agg(e1, ad);
+      cvt_agg(b0, bd);
+      combine_into<double, NX, true>(ad, bd);
+      cvt_agg(bd, e2);
+      count_fallback(fb);
+    }
+  } else {
+    combine_into<R, NX, SQRT>(e1, e2);
+  }
+}
+
 }  // namespace psm
+

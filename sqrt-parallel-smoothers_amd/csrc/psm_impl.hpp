@@ -244,8 +244,11 @@ struct Impl {
     return hip_check("k_smoother_combine");
   }
 
+  // Exclusive scan of the level-0 aggregates; *final_off receives the workspace
+  // offset of the complete exclusive prefixes.
   template <typename Agg>
-  static int run_scan(const Plan& pl, char* ws, int reverse, hipStream_t st) {
+  static int run_scan(const Plan& pl, char* ws, int reverse, const Fallback& fb, hipStream_t st,
+                      size_t* final_off) {
     const bool filt = (Agg::NQ == FA::NQ) && !reverse;
     for (int l = 0; l < pl.nlevels; ++l) {
       const ScanLevel& s = pl.lv[l];
@@ -255,19 +258,23 @@ struct Impl {
       TimedScope ts_(l == 0 ? (filt ? "k_fscan_up_l0" : "k_sscan_up_l0")
                             : (filt ? "k_fscan_up" : "k_sscan_up"), st);
       hipLaunchKernelGGL((k_scan_up<Agg, R, SQRT>), dim3((unsigned)s.nwg), dim3(kScanWG), 0, st,
-                         (const R*)in, s.n, s.K, (l == 0) ? reverse : 0, out, tot, s.nwg);
+                         (const R*)in, s.n, s.K, (l == 0) ? reverse : 0, out, tot, s.nwg, fb);
       int rc = hip_check("k_scan_up");
       if (rc) return rc;
     }
+    // complete prefixes of level l land in lv[l].in (free once the up-sweep is done)
     for (int l = pl.nlevels - 2; l >= 0; --l) {
       const ScanLevel& s = pl.lv[l];
+      const size_t upper_off =
+          (l + 1 == pl.nlevels - 1) ? pl.lv[l + 1].off_out : pl.lv[l + 1].off_in;
       TimedScope ts_(filt ? "k_fscan_down" : "k_sscan_down", st);
       hipLaunchKernelGGL((k_scan_down<Agg, R, SQRT>), dim3((unsigned)s.nwg), dim3(kScanWG), 0, st,
-                         (R*)(ws + s.off_out), s.n, s.K, (l == 0) ? reverse : 0,
-                         (const R*)(ws + pl.lv[l + 1].off_out), pl.lv[l + 1].n);
+                         (const R*)(ws + s.off_out), s.n, s.K, (l == 0) ? reverse : 0,
+                         (const R*)(ws + upper_off), pl.lv[l + 1].n, (R*)(ws + s.off_in), fb);
       int rc = hip_check("k_scan_down");
       if (rc) return rc;
     }
+    *final_off = pl.nlevels == 1 ? pl.lv[0].off_out : pl.lv[0].off_in;
     return PSM_OK;
   }
 
@@ -301,22 +308,24 @@ struct Impl {
       if (rc) return rc;
     }
     const R* lin = (const R*)(ws + pl.off_lin);
+    const Fallback fb{p->fallback_ratio, p->fallback_count};
     {
     TimedScope ts_("k_filter_reduce", st);
     hipLaunchKernelGGL((k_filter_reduce<R, NX, NY, SQRT>), dim3(grid_for(pl.nchunks, kLaneWG)),
                        dim3(kLaneWG), 0, st, pl.M, pl.chunk, pl.nchunks, p->state_offsets,
                        p->n_seq, lin, (const R*)ys, (const R*)m0, (const R*)L0,
-                       (R*)(ws + pl.lv[0].off_in));
+                       (R*)(ws + pl.lv[0].off_in), fb);
     }
     if ((rc = hip_check("k_filter_reduce"))) return rc;
-    if ((rc = run_scan<FA>(pl, ws, 0, st))) return rc;
+    size_t pre_off = 0;
+    if ((rc = run_scan<FA>(pl, ws, 0, fb, st, &pre_off))) return rc;
     double* ell_state = ell ? (double*)(ws + pl.off_ell) : nullptr;
     {
     TimedScope ts_("k_filter_apply", st);
     hipLaunchKernelGGL((k_filter_apply<R, NX, NY, SQRT>), dim3(grid_for(pl.nchunks, kLaneWG)),
                        dim3(kLaneWG), 0, st, pl.M, pl.chunk, pl.nchunks, p->state_offsets,
                        p->n_seq, lin, (const R*)ys, (const R*)m0, (const R*)L0,
-                       (const R*)(ws + pl.lv[0].off_out), (R*)om, (R*)oL, ell_state);
+                       (const R*)(ws + pre_off), (R*)om, (R*)oL, ell_state, fb);
     }
     if ((rc = hip_check("k_filter_apply"))) return rc;
     if (ell) {
@@ -342,21 +351,23 @@ struct Impl {
       rc = linearize_states(pl, tr, nullptr, sg, nm, nL, ws, st);
       if (rc) return rc;
     }
+    const Fallback fb{p->fallback_ratio, p->fallback_count};
     {
     TimedScope ts_("k_smoother_reduce", st);
     hipLaunchKernelGGL((k_smoother_reduce<R, NX, NY, SQRT>), dim3(grid_for(pl.nchunks, kLaneWG)),
                        dim3(kLaneWG), 0, st, pl.M, pl.chunk, pl.nchunks, p->state_offsets,
                        p->n_seq, (const R*)(ws + pl.off_lin), (const R*)fm, (const R*)fL,
-                       (R*)(ws + pl.off_selem), (R*)(ws + pl.lv[0].off_in));
+                       (R*)(ws + pl.off_selem), (R*)(ws + pl.lv[0].off_in), fb);
     }
     if ((rc = hip_check("k_smoother_reduce"))) return rc;
-    if ((rc = run_scan<SA>(pl, ws, 1, st))) return rc;
+    size_t pre_off = 0;
+    if ((rc = run_scan<SA>(pl, ws, 1, fb, st, &pre_off))) return rc;
     {
       TimedScope ts_("k_smoother_apply", st);
       hipLaunchKernelGGL((k_smoother_apply<R, NX, SQRT>), dim3(grid_for(pl.nchunks, kLaneWG)),
                          dim3(kLaneWG), 0, st, pl.M, pl.chunk, pl.nchunks,
-                         (const R*)(ws + pl.off_selem), (const R*)(ws + pl.lv[0].off_out), (R*)om,
-                         (R*)oL);
+                         (const R*)(ws + pl.off_selem), (const R*)(ws + pre_off), (R*)om,
+                         (R*)oL, fb);
     }
     return hip_check("k_smoother_apply");
   }

@@ -133,7 +133,7 @@ template <typename R, int NX, int NY, bool SQRT>
 __global__ __launch_bounds__(kLaneWG) void k_filter_reduce(
     int64_t M, int chunk, int64_t nchunks, const int64_t* __restrict__ soff, int nseq,
     const R* __restrict__ lin, const R* __restrict__ ys, const R* __restrict__ m0,
-    const R* __restrict__ L0, R* __restrict__ agg) {
+    const R* __restrict__ L0, R* __restrict__ agg, Fallback fb) {
   const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= nchunks) return;
   const int64_t i0 = c * chunk;
@@ -156,7 +156,7 @@ __global__ __launch_bounds__(kLaneWG) void k_filter_reduce(
       load_obs(lin, M, i, st_);
       R y[NY];
       ld_aos(ys, i - s - 1, y);
-      absorb<R, NX, NY, SQRT>(a, st_, y);
+      absorb_fb<R, NX, NY, SQRT>(a, st_, y, fb);
     }
   }
   store_agg(agg, nchunks, c, a);
@@ -175,7 +175,8 @@ __global__ __launch_bounds__(kLaneWG) void k_filter_reduce(
 template <typename Agg, typename R, bool SQRT>
 __global__ __launch_bounds__(kScanWG) void k_scan_up(const R* __restrict__ in, int64_t n, int K,
                                                      int reverse, R* __restrict__ out_excl,
-                                                     R* __restrict__ totals, int64_t ntot) {
+                                                     R* __restrict__ totals, int64_t ntot,
+                                                     Fallback fb) {
   using V = typename ViewOf<Agg>::type;
   constexpr int NX = Agg_nx<Agg>::value;
   constexpr int NQ = Agg::NQ;
@@ -190,14 +191,14 @@ __global__ __launch_bounds__(kScanWG) void k_scan_up(const R* __restrict__ in, i
     const int64_t p = base + j;
     if (p >= n) break;
     const V e{inm, n, reverse ? n - 1 - p : p};
-    combine_mem<R, NX, SQRT>(slot(cur, t), e, slot(1 - cur, t));
+    combine_mem_fb<R, NX, SQRT>(slot(cur, t), e, slot(1 - cur, t), fb);
     cur = 1 - cur;
   }
   if (cur == 1) view_copy(slot(1, t), slot(0, t));
   cur = 0;
   for (int d = 1; d < kScanWG; d <<= 1) {
     __syncthreads();
-    if (t >= d) combine_mem<R, NX, SQRT>(slot(cur, t - d), slot(cur, t), slot(1 - cur, t));
+    if (t >= d) combine_mem_fb<R, NX, SQRT>(slot(cur, t - d), slot(cur, t), slot(1 - cur, t), fb);
     else view_copy(slot(cur, t), slot(1 - cur, t));
     cur = 1 - cur;
   }
@@ -214,26 +215,32 @@ __global__ __launch_bounds__(kScanWG) void k_scan_up(const R* __restrict__ in, i
     const int64_t item = reverse ? n - 1 - p : p;
     const V o{out_excl, n, item};
     view_copy(run, o);
-    combine_mem<R, NX, SQRT>(o, V{inm, n, item}, run);
+    combine_mem_fb<R, NX, SQRT>(o, V{inm, n, item}, run, fb);
   }
 }
 
-// out_excl[item] <- upper[w] (+) out_excl[item] for every item of workgroup w > 0.
+// dst[item] <- upper[w] (+) out_excl[item] (workgroup w > 0; w == 0 copies):
+// the complete exclusive prefixes, written out of place so the operands stay
+// intact for the FP64 fallback.
 template <typename Agg, typename R, bool SQRT>
-__global__ __launch_bounds__(kScanWG) void k_scan_down(R* __restrict__ out_excl, int64_t n, int K,
-                                                       int reverse, const R* __restrict__ upper,
-                                                       int64_t nupper) {
+__global__ __launch_bounds__(kScanWG) void k_scan_down(const R* __restrict__ out_excl, int64_t n,
+                                                       int K, int reverse,
+                                                       const R* __restrict__ upper,
+                                                       int64_t nupper, R* __restrict__ dst,
+                                                       Fallback fb) {
   using V = typename ViewOf<Agg>::type;
   constexpr int NX = Agg_nx<Agg>::value;
-  if (blockIdx.x == 0) return;
   const int t = threadIdx.x;
   const int64_t base = ((int64_t)blockIdx.x * kScanWG + t) * K;
   const V up{const_cast<R*>(upper), nupper, (int64_t)blockIdx.x};
   for (int j = 0; j < K; ++j) {
     const int64_t p = base + j;
     if (p >= n) break;
-    const V e{out_excl, n, reverse ? n - 1 - p : p};
-    combine_mem<R, NX, SQRT>(up, e, e);
+    const int64_t item = reverse ? n - 1 - p : p;
+    const V e{const_cast<R*>(out_excl), n, item};
+    const V o{dst, n, item};
+    if (blockIdx.x == 0) view_copy(e, o);
+    else combine_mem_fb<R, NX, SQRT>(up, e, o, fb);
   }
 }
 
@@ -245,7 +252,7 @@ __global__ __launch_bounds__(kLaneWG) void k_filter_apply(
     int64_t M, int chunk, int64_t nchunks, const int64_t* __restrict__ soff, int nseq,
     const R* __restrict__ lin, const R* __restrict__ ys, const R* __restrict__ m0,
     const R* __restrict__ L0, const R* __restrict__ prefix, R* __restrict__ out_m,
-    R* __restrict__ out_L, double* __restrict__ ell_state) {
+    R* __restrict__ out_L, double* __restrict__ ell_state, Fallback fb) {
   const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= nchunks) return;
   const int64_t i0 = c * chunk;
@@ -270,7 +277,7 @@ __global__ __launch_bounds__(kLaneWG) void k_filter_apply(
       load_obs(lin, M, i, st_);
       R y[NY];
       ld_aos(ys, i - s - 1, y);
-      e = kf_step<R, NX, NY, SQRT>(st_, y, m, L);
+      e = kf_step_fb<R, NX, NY, SQRT>(st_, y, m, L, fb);
     }
     st_aos(out_m, i, m);
     st_aos(out_L, i, L);
@@ -304,7 +311,7 @@ template <typename R, int NX, int NY, bool SQRT>
 __global__ __launch_bounds__(kLaneWG) void k_smoother_reduce(
     int64_t M, int chunk, int64_t nchunks, const int64_t* __restrict__ soff, int nseq,
     const R* __restrict__ lin, const R* __restrict__ fm, const R* __restrict__ fL,
-    R* __restrict__ elems, R* __restrict__ agg) {
+    R* __restrict__ elems, R* __restrict__ agg, Fallback fb) {
   const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= nchunks) return;
   const int64_t i0 = c * chunk;
@@ -329,10 +336,10 @@ __global__ __launch_bounds__(kLaneWG) void k_smoother_reduce(
       ld(lin, M, i, q, F);
       ld(lin, M, i, q, cq);
       ld(lin, M, i, q, bt);
-      smoother_element<R, NX, SQRT>(F, cq, bt, m, L, e);
+      smoother_element_fb<R, NX, SQRT>(F, cq, bt, m, L, e, fb);
     }
     store_agg(elems, M, i, e);
-    combine_into<R, NX, SQRT>(acc, e);
+    combine_into_fb<R, NX, SQRT>(acc, e, fb);
     copy_agg(e, acc);
   }
   store_agg(agg, nchunks, c, acc);
@@ -341,7 +348,7 @@ __global__ __launch_bounds__(kLaneWG) void k_smoother_reduce(
 template <typename R, int NX, bool SQRT>
 __global__ __launch_bounds__(kLaneWG) void k_smoother_apply(
     int64_t M, int chunk, int64_t nchunks, const R* __restrict__ elems,
-    const R* __restrict__ prefix, R* __restrict__ out_m, R* __restrict__ out_L) {
+    const R* __restrict__ prefix, R* __restrict__ out_m, R* __restrict__ out_L, Fallback fb) {
   const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= nchunks) return;
   const int64_t i0 = c * chunk;
@@ -360,7 +367,7 @@ __global__ __launch_bounds__(kLaneWG) void k_smoother_apply(
       copy(e.g, m);
       copy(e.D, D);
     } else {
-      smoother_apply<R, NX, SQRT>(e, m, D);
+      smoother_apply_fb<R, NX, SQRT>(e, m, D, fb);
     }
     st_aos(out_m, i, m);
     st_aos(out_L, i, D);

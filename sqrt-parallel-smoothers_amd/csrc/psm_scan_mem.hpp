@@ -373,3 +373,102 @@ template <typename R, int NX> struct ViewOf<FAgg<R, NX>> { using type = FView<R,
 template <typename R, int NX> struct ViewOf<SAgg<R, NX>> { using type = SView<R, NX>; };
 
 }  // namespace psm
+
+namespace psm {
+
+template <typename R, int NX>
+PSM_DEV void view_load_d(const FView<R, NX>& v, FAgg<double, NX>& a) {
+#pragma unroll
+  for (int i = 0; i < NX; ++i) {
+    a.b[i] = v.b(i);
+    a.eta[i] = v.eta(i);
+#pragma unroll
+    for (int j = 0; j < NX; ++j) {
+      a.A[i][j] = v.A(i, j);
+      a.U[i][j] = v.U(i, j);
+      a.Z[i][j] = v.Z(i, j);
+    }
+  }
+  a.flag = v.flag();
+}
+template <typename R, int NX>
+PSM_DEV void view_store_d(const FAgg<double, NX>& a, const FView<R, NX>& v) {
+#pragma unroll
+  for (int i = 0; i < NX; ++i) {
+    v.b(i) = (R)a.b[i];
+    v.eta(i) = (R)a.eta[i];
+#pragma unroll
+    for (int j = 0; j < NX; ++j) {
+      v.A(i, j) = (R)a.A[i][j];
+      v.U(i, j) = (R)a.U[i][j];
+      v.Z(i, j) = (R)a.Z[i][j];
+    }
+  }
+}
+template <typename R, int NX>
+PSM_DEV void view_load_d(const SView<R, NX>& v, SAgg<double, NX>& a) {
+#pragma unroll
+  for (int i = 0; i < NX; ++i) {
+    a.g[i] = v.g(i);
+#pragma unroll
+    for (int j = 0; j < NX; ++j) {
+      a.E[i][j] = v.E(i, j);
+      a.D[i][j] = v.D(i, j);
+    }
+  }
+  a.flag = v.flag();
+}
+template <typename R, int NX>
+PSM_DEV void view_store_d(const SAgg<double, NX>& a, const SView<R, NX>& v) {
+#pragma unroll
+  for (int i = 0; i < NX; ++i) {
+    v.g(i) = (R)a.g[i];
+#pragma unroll
+    for (int j = 0; j < NX; ++j) {
+      v.E(i, j) = (R)a.E[i][j];
+      v.D(i, j) = (R)a.D[i][j];
+    }
+  }
+}
+
+template <typename R, int NX>
+PSM_DEV bool view_ill(const FView<R, NX>& v, float thr) {
+  R L[NX][NX];
+#pragma unroll
+  for (int i = 0; i < NX; ++i)
+#pragma unroll
+    for (int j = 0; j < NX; ++j) L[i][j] = (i == j) ? v.U(i, i) : R(0);
+  return ill_conditioned(L, thr);
+}
+template <typename R, int NX>
+PSM_DEV bool view_ill(const SView<R, NX>& v, float thr) {
+  R L[NX][NX];
+#pragma unroll
+  for (int i = 0; i < NX; ++i)
+#pragma unroll
+    for (int j = 0; j < NX; ++j) L[i][j] = (i == j) ? v.D(i, i) : R(0);
+  return ill_conditioned(L, thr);
+}
+
+// combine_mem with the FP64 re-triangularisation fallback (FP32 square-root
+// form).  out must not alias e2 here, so the operands survive for the redo.
+template <typename R, int NX, bool SQRT, typename V>
+PSM_DEV void combine_mem_fb(const V& e1, const V& e2, const V& out, const Fallback& fb) {
+  const int f1 = e1.flag(), f2 = e2.flag();
+  combine_mem<R, NX, SQRT>(e1, e2, out);
+  if constexpr (kFallback<R, SQRT>) {
+    if (fb.thr > 0.f && f2 != FLAG_HEAD && f1 != FLAG_IDENT && f2 != FLAG_IDENT &&
+        view_ill(out, fb.thr)) {
+      using AD = typename std::conditional<std::is_same<V, FView<R, NX>>::value, FAgg<double, NX>,
+                                           SAgg<double, NX>>::type;
+      AD a, b;
+      view_load_d(e1, a);
+      view_load_d(e2, b);
+      combine_into<double, NX, true>(a, b);
+      view_store_d(b, out);
+      count_fallback(fb);
+    }
+  }
+}
+
+}  // namespace psm

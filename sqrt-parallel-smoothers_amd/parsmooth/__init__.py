@@ -4,5 +4,6 @@ iterated smoothing (IEKS / IPLS) with the public API of the reference library
 (libparsmooth_hip.so) on one GPU."""
 from ._base import MVNStandard, MVNSqrt, FunctionalModel, ConditionalMomentsModel
 from .methods import filtering, smoothing, filter_smoother, iterated_smoothing, sampling
+from ._engine import set_fp32_fallback
 
 __version__ = "1.0.0"

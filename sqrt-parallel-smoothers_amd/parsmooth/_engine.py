@@ -14,6 +14,18 @@ from ._base import MVNSqrt, MVNStandard
 from .linearization._methods import Method, model_struct
 
 
+# Default threshold of min|diag| / max|diag| below which an FP32 square-root
+# step is recomputed in FP64 (see include/psm.h, psm_problem.fallback_ratio).
+FP32_FALLBACK_RATIO = 1e-4
+
+
+def set_fp32_fallback(ratio: float):
+    """Set the FP64 re-triangularisation threshold for FP32 square-root runs
+    created afterwards (0 disables the fallback)."""
+    global FP32_FALLBACK_RATIO
+    FP32_FALLBACK_RATIO = float(ratio)
+
+
 def to_dev(x, dtype, dev):
     if isinstance(x, torch.Tensor):
         return x.to(device=dev, dtype=dtype).contiguous()
@@ -36,7 +48,7 @@ class Problem:
     ``lengths[s] + 1`` states (its prior state first)."""
 
     def __init__(self, nx: int, ny: int, lengths: Sequence[int], dtype: torch.dtype, sqrt: bool,
-                 dev, chunk: int = 0):
+                 dev, chunk: int = 0, tangent_code: Optional[int] = None):
         lengths = np.asarray(lengths, dtype=np.int64).reshape(-1)
         if lengths.size == 0 or np.any(lengths < 1):
             raise ValueError("every sequence needs at least one observation")
@@ -52,12 +64,16 @@ class Problem:
         self.sqrt = sqrt
         self.dev = dev
         self.state_offsets = torch.as_tensor(offs).to(dev)
-        if not N.lib().psm_supported(N.dtype_code(dtype), N.FORM_SQRT if sqrt else N.FORM_STD,
-                                     self.nx, self.ny):
-            raise NotImplementedError(f"no compiled kernels for nx={nx}, ny={ny}, dtype={dtype}")
-        self.struct = N.PsmProblem(N.dtype_code(dtype), N.FORM_SQRT if sqrt else N.FORM_STD,
+        code = N.dtype_code(dtype) if tangent_code is None else int(tangent_code)
+        if not N.lib().psm_supported(code, N.FORM_SQRT if sqrt else N.FORM_STD, self.nx, self.ny):
+            raise NotImplementedError(f"no compiled kernels for nx={nx}, ny={ny}, dtype code {code}")
+        # FP64 re-triangularisation fallback of the FP32 square-root path
+        self.fallback_counter = torch.zeros(1, dtype=torch.int64, device=dev)
+        ratio = FP32_FALLBACK_RATIO if (code == N.PSM_F32 and sqrt) else 0.0
+        self.struct = N.PsmProblem(code, N.FORM_SQRT if sqrt else N.FORM_STD,
                                    self.nx, self.ny, self.n_seq, int(chunk), self.n_states,
-                                   self.state_offsets.data_ptr())
+                                   self.state_offsets.data_ptr(), float(ratio), 0,
+                                   self.fallback_counter.data_ptr())
         self.ws_bytes = int(N.lib().psm_workspace_bytes(ctypes.byref(self.struct)))
         if self.ws_bytes == 0:
             N.check(-1, "psm_workspace_bytes")

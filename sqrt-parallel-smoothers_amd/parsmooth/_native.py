@@ -14,6 +14,7 @@ import threading
 import torch
 
 PSM_F32, PSM_F64 = 0, 1
+PSM_F32_TANGENT, PSM_F64_TANGENT = 2, 3
 FORM_SQRT, FORM_STD = 0, 1
 
 MODEL_LINEAR = 0
@@ -50,7 +51,9 @@ class PsmSigma(ctypes.Structure):
 class PsmProblem(ctypes.Structure):
     _fields_ = [("dtype", ctypes.c_int32), ("form", ctypes.c_int32), ("nx", ctypes.c_int32),
                 ("ny", ctypes.c_int32), ("n_seq", ctypes.c_int32), ("chunk", ctypes.c_int32),
-                ("n_states", ctypes.c_int64), ("state_offsets", ctypes.c_void_p)]
+                ("n_states", ctypes.c_int64), ("state_offsets", ctypes.c_void_p),
+                ("fallback_ratio", ctypes.c_float), ("pad_", ctypes.c_int32),
+                ("fallback_count", ctypes.c_void_p)]
 
 
 _lib = None

@@ -50,6 +50,10 @@ class IteratedSmoother:
         self.m0 = None
         self.L0 = None
 
+    def fallback_count(self) -> int:
+        """Number of FP64 recomputations since construction (FP32 square-root form)."""
+        return int(self.prob.fallback_counter.item())
+
     def chunk_size(self) -> int:
         """States per lane in the chunk passes (as resolved by the library)."""
         import ctypes

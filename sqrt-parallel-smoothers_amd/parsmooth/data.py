@@ -5,6 +5,9 @@
   follows 1 + q dt cumsum(randn); the state is propagated with the matrix
   exponential of the 4x4 turn generator).
 * ``lgssm_data``: linear-Gaussian simulation, tests/_lgssm.py:42-95.
+* ``bearings_data_batch``: many such targets at once (uneven lengths), with the
+  turn propagated by the closed form of the same matrix exponential
+  (the velocity rotates by -a dt and the position integrates it).
 * ``cv_model``: the nx=4 constant-velocity tracking model (white acceleration).
 """
 from __future__ import annotations
@@ -65,3 +68,32 @@ def cv_model(dt=0.1, q=1.0, r=0.5):
     H = np.eye(2, 4)
     R = r ** 2 * np.eye(2)
     return F, Q, H, R
+
+
+def bearings_data_batch(lengths, x0s, dt, r, s1, s2, q=10.0, seed=0):
+    """Simulate len(lengths) bearings-only targets; returns (true_states list,
+    observations list) with float32 arrays of shapes (T_s+1, 5) and (T_s, 2).
+    x0s: (n_seq, 4) initial positions/velocities."""
+    rng = np.random.default_rng(seed)
+    lengths = np.asarray(lengths, dtype=np.int64)
+    B, Tm = lengths.size, int(lengths.max())
+    a = (1 + q * dt * np.cumsum(rng.standard_normal((B, Tm)), axis=1)).astype(np.float32)
+    noise = rng.standard_normal((B, Tm, 2)).astype(np.float32)
+    x = np.asarray(x0s, dtype=np.float64).reshape(B, 4).copy()
+    states = np.zeros((B, Tm + 1, 5), dtype=np.float32)
+    obs = np.zeros((B, Tm, 2), dtype=np.float32)
+    states[:, 0, :4] = x
+    s1 = np.asarray(s1, dtype=np.float64)
+    s2 = np.asarray(s2, dtype=np.float64)
+    for k in range(Tm):
+        w = a[:, k].astype(np.float64)
+        c, s = np.cos(w * dt), np.sin(w * dt)
+        sw, cw = s / w, (1.0 - c) / w
+        px, py, vx, vy = x[:, 0], x[:, 1], x[:, 2], x[:, 3]
+        x = np.stack([px + sw * vx + cw * vy, py - cw * vx + sw * vy,
+                      c * vx + s * vy, -s * vx + c * vy], axis=1)
+        obs[:, k, 0] = np.arctan2(x[:, 1] - s1[1], x[:, 0] - s1[0]) + r * noise[:, k, 0]
+        obs[:, k, 1] = np.arctan2(x[:, 1] - s2[1], x[:, 0] - s2[0]) + r * noise[:, k, 1]
+        states[:, k + 1, :4] = x
+        states[:, k + 1, 4] = w
+    return ([states[b, :lengths[b] + 1] for b in range(B)], [obs[b, :lengths[b]] for b in range(B)])

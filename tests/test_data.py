@@ -1,0 +1,44 @@
+"""Input generators (CPU)."""
+import numpy as np
+import scipy.linalg as la
+
+from parsmooth.data import bearings_data, bearings_data_batch, lgssm_data
+
+
+def _turn(a, dt):
+    G = np.array([[0, 0, 1, 0], [0, 0, 0, 1], [0, 0, 0, a], [0, 0, -a, 0]], dtype=np.float64)
+    return la.expm(G * dt)
+
+
+def test_batch_simulator_follows_the_matrix_exponential():
+    lengths = [5, 17, 3]
+    states, obs = bearings_data_batch(lengths, np.tile([0.1, 0.2, 1.0, 0.0], (3, 1)), 0.01, 0.5,
+                                      (-1.5, 0.5), (1.0, 1.0), seed=3)
+    for st, ob, T in zip(states, obs, lengths):
+        assert st.shape == (T + 1, 5) and ob.shape == (T, 2)
+        for k in range(T):
+            pred = _turn(float(st[k + 1, 4]), 0.01) @ st[k, :4].astype(np.float64)
+            np.testing.assert_allclose(st[k + 1, :4], pred, atol=1e-5)
+
+
+def test_single_simulator_matches_reference_recipe():
+    """notebooks/bearing_data.py:154-198: same RandomState draws and recipe."""
+    T = 20
+    rs = np.random.RandomState(7)
+    a_s = (1 + 10.0 * 0.01 * np.cumsum(rs.randn(T))).astype(np.float32)
+    normals = rs.randn(T, 2).astype(np.float32)
+    _, states, ys = bearings_data(np.array([0.1, 0.2, 1.0, 0.0]), 0.01, 0.5, T, (-1.5, 0.5), (1.0, 1.0),
+                                  random_state=7)
+    x = np.array([0.1, 0.2, 1.0, 0.0], dtype=np.float32)
+    for i, a in enumerate(a_s):
+        x = _turn(float(a), 0.01).astype(np.float32) @ x
+        np.testing.assert_allclose(states[i + 1, :4], x, atol=1e-6)
+        np.testing.assert_allclose(ys[i, 0], np.arctan2(x[1] - 0.5, x[0] + 1.5) + 0.5 * normals[i, 0], atol=1e-6)
+
+
+def test_lgssm_shapes_and_determinism():
+    A = np.eye(2)
+    s1, y1 = lgssm_data(np.zeros(2), A, np.eye(1, 2), np.eye(1), np.eye(2), np.zeros(2), np.zeros(1), 10, random_state=0)
+    s2, y2 = lgssm_data(np.zeros(2), A, np.eye(1, 2), np.eye(1), np.eye(2), np.zeros(2), np.zeros(1), 10, random_state=0)
+    assert y1.shape == (10, 1) and s1.shape == (11, 2)
+    np.testing.assert_array_equal(y1, y2)

@@ -27,6 +27,7 @@ from parsmooth.parallel._smoothing import _sqrt_associative_params, _standard_as
 
 DEV = torch.device("cuda", 0)
 TOL64 = 1e-9
+TOL32 = 1e-4  # FP32 arithmetic against the FP64 oracle (relative to the largest magnitude)
 
 
 def t64(a):
@@ -395,3 +396,46 @@ def test_fp32_bearings_vs_fp64_oracle():
                                        criterion=lambda i, *_: i < 5)
     assert got.mean.dtype == torch.float32
     close(npy(got.mean), ref.mean, tol=1e-4, what="fp32 mean")
+
+
+def test_fp32_fallback_recomputes_in_fp64():
+    """FP32 square-root pass at an identical nominal.  With the re-triangularisation
+    threshold forced to 1 every check fires and the FP64 recomputation path runs
+    (counter > 0); with the check off nothing is counted.  Both results match the
+    FP64 oracle evaluated on the same FP32-rounded inputs to the FP32 tolerance,
+    and recomputing in FP64 does not make the result worse."""
+    from parsmooth import set_fp32_fallback
+    from parsmooth.batch import IteratedSmoother
+    T = 1500
+    ys, _, (td64, od64) = _bearings_setup(T, seed=9)
+    r32 = lambda a: np.asarray(a, dtype=np.float32).astype(np.float64)
+    ys = r32(ys)
+    m0 = r32([0.1, 0.2, 1.0, 0.0, 0.0])
+    rng = np.random.RandomState(1)
+    nom_m = r32(np.cumsum(0.01 * rng.randn(T + 1, 5), 0) + np.array([0.1, 0.2, 1.0, 0.0, 0.5]))
+    nom_c = r32(np.repeat(np.eye(5)[None] * 0.1, T + 1, 0))
+    cQ, cR = r32(npy(td64.mvn.chol)), r32(npy(od64.mvn.chol))
+    to = O.FunctionalModel(O.CoordinatedTurn(float(np.float32(0.01))), O.MVNSqrt(np.zeros(5), cQ))
+    oo = O.FunctionalModel(O.Bearings((-1.5, 0.5), (1.0, 1.0)), O.MVNSqrt(np.zeros(2), cR))
+    ref = O.filter_smoother(ys, O.MVNSqrt(m0, np.eye(5)), to, oo, O.extended, O.MVNSqrt(nom_m, nom_c), True)
+    t32 = lambda a: torch.as_tensor(np.asarray(a), dtype=torch.float32, device=DEV)
+    td = FunctionalModel(td64.function, MVNSqrt(t32(np.zeros(5)), t32(cQ)))
+    od = FunctionalModel(od64.function, MVNSqrt(t32(np.zeros(2)), t32(cR)))
+    errs, counts = [], []
+    try:
+        for ratio in (0.0, 1.0):
+            set_fp32_fallback(ratio)
+            r = IteratedSmoother([T], td, od, extended, dtype=torch.float32)
+            r.set_data(t32(ys), t32(m0[None]), t32(np.eye(5)[None]))
+            out = r.prob.new_traj()
+            r.pass_once((t32(nom_m), t32(nom_c)), out)
+            torch.cuda.synchronize()
+            close(npy(out[0]), ref.mean, tol=TOL32, what=f"fp32 mean, fallback ratio {ratio}")
+            close(gram(npy(out[1])), gram(ref.chol), tol=TOL32, what=f"fp32 cov, fallback ratio {ratio}")
+            errs.append(np.abs(npy(out[0]) - ref.mean).max())
+            counts.append(r.fallback_count())
+    finally:
+        set_fp32_fallback(1e-4)
+    print("fp32 mean errors (no fallback, all recomputed):", errs, "fallback counts:", counts)
+    assert counts[0] == 0 and counts[1] > 0
+    assert errs[1] <= 1.5 * errs[0]
