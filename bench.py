@@ -43,6 +43,11 @@ PEAK_FP32 = 157.3e12
 PEAK_HBM = 8.0e12
 
 
+def log(msg):
+    """Progress line on stderr (the JSON result is the only stdout line)."""
+    print(f"[bench {time.strftime('%H:%M:%S')}] {msg}", file=sys.stderr, flush=True)
+
+
 def make_inputs(T, seed=0):
     from parsmooth.data import bearings_data
     _, _, ys = bearings_data(np.array([0.1, 0.2, 1.0, 0.0]), 0.01, 0.5, T, (-1.5, 0.5), (1.0, 1.0),
@@ -91,6 +96,40 @@ def build_batch_runner(n_seq, lo, hi, seed, dtype, method):
     reps = lengths + 1
     init = (t(np.repeat(m0, reps, axis=0)), t(np.tile(np.eye(5), (int(reps.sum()), 1, 1))))
     return runner, init, int(lengths.sum())
+
+
+def build_grad_objective(n_seq, T, n_iter, dtype, seed=5):
+    """Pseudo-log-likelihood of cubature IPLS for n_seq bearings-only sequences of
+    length T simulated at qc=0.01, qw=0.1, sigma=0.5, and its gradient in
+    (log qc, log qw, log sigma, s1x, s1y, s2x, s2y)."""
+    import torch
+    from parsmooth import _native as N
+    from parsmooth.data import bearings_data_batch
+    from parsmooth.gradient import IteratedLoglikelihood, bearings_parameterisation
+    from parsmooth.linearization import cubature
+    rng = np.random.default_rng(seed)
+    m0 = np.array([0.1, 0.2, 1.0, 0.0, 0.0]) + np.sqrt(0.1) * rng.standard_normal((n_seq, 5))
+    lengths = np.full(n_seq, T)
+    _, obs = bearings_data_batch(lengths, m0[:, :4], 0.01, 0.5, (-1.5, 0.5), (1.0, 1.0), seed=seed + 1)
+    L0 = np.tile(np.eye(5), (n_seq, 1, 1))
+    build = bearings_parameterisation(0.01, m0, L0)
+    init = (np.repeat(m0, T + 1, axis=0), np.tile(np.eye(5), (n_seq * (T + 1), 1, 1)))
+    obj = IteratedLoglikelihood(build, N.MODEL_COORDINATED_TURN, N.MODEL_BEARINGS, 5, 2, lengths,
+                                np.concatenate(obs), cubature, n_iter, init, dtype=dtype)
+    theta = torch.tensor([np.log(0.01), np.log(0.1), np.log(0.5), -1.5, 0.5, 1.0, 1.0], dtype=torch.float64)
+    return obj, theta
+
+
+def time_gradient(obj, theta, steps, warmup):
+    import torch
+    for _ in range(warmup):
+        obj.value_and_grad(theta)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        ell, grad = obj.value_and_grad(theta)
+    torch.cuda.synchronize()
+    return (time.perf_counter() - t0) / steps, ell, grad
 
 
 def time_runner(runner, init, n_iter, steps, warmup, use_graph, kernel_timing=False):
@@ -158,6 +197,9 @@ def main():
     ap.add_argument("--cpu-repeats", type=int, default=4)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--batch-seqs", type=int, default=4096)
+    ap.add_argument("--grad-seqs", type=int, default=256)
+    ap.add_argument("--grad-T", type=int, default=16384)
+    ap.add_argument("--grad-iter", type=int, default=10)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run one 256 MiB device copy first (byte-counter calibration)")
     args = ap.parse_args()
@@ -181,12 +223,14 @@ def main():
     ys = make_inputs(T)
 
     # FP64 (headline), with live per-kernel HIP-event timing in a separate pass
+    log(f"FP64 IEKS T={T} n_iter={n_iter}")
     r64, init64 = build_runner(T, ys, torch.float64)
     dt64, out64, _ = time_runner(r64, init64, n_iter, args.steps, args.warmup, bool(args.graph))
     dtk, _, rep = time_runner(r64, init64, n_iter, max(1, args.steps // 2), 0, False, kernel_timing=True)
     mean64 = out64[0].double().cpu().numpy()
 
     # FP32
+    log("FP32 IEKS")
     r32, init32 = build_runner(T, ys, torch.float32)
     r32.prob.fallback_counter.zero_()
     dt32, out32, _ = time_runner(r32, init32, n_iter, args.steps, args.warmup, bool(args.graph))
@@ -207,6 +251,7 @@ def main():
     batch = None
     if args.batch_seqs > 0:
         from parsmooth.linearization import cubature
+        log(f"batch leg: {args.batch_seqs} sequences")
         rb, ib, nsteps = build_batch_runner(args.batch_seqs, 256, 8192, 0, torch.float32, cubature)
         dtb, _, repb = time_runner(rb, ib, 5, max(1, args.steps // 2), 1, bool(args.graph))
         _, _, repb = time_runner(rb, ib, 5, 1, 0, False, kernel_timing=True)
@@ -216,6 +261,31 @@ def main():
                  "unit": "step-it/s", "kernel_ms": {k: v[0] / v[1] for k, v in repb.items()}}
         del rb, ib
         torch.cuda.empty_cache()
+
+    grad = None
+    if args.grad_seqs > 0:
+        legs = {}
+        for name, dt_ in (("fp32", torch.float32), ("fp64", torch.float64)):
+            log(f"gradient leg {name}: {args.grad_seqs} x T={args.grad_T}")
+            obj, theta = build_grad_objective(args.grad_seqs, args.grad_T, args.grad_iter, dt_)
+            tg, ell_g, g_g = time_gradient(obj, theta, 1, 1 if name == "fp32" else 0)
+            legs[name] = (tg, ell_g.double().cpu().numpy(), g_g.double().cpu().numpy())
+            del obj
+            torch.cuda.empty_cache()
+        g32, g64 = legs["fp32"][2].sum(0), legs["fp64"][2].sum(0)
+        nst = args.grad_seqs * args.grad_T
+        grad = {"workload": f"{args.grad_seqs} bearings-only sequences x T={args.grad_T}, cubature IPLS, "
+                            f"n_iter={args.grad_iter}, gradient of the summed pseudo-log-likelihood in 7 "
+                            "parameters (implicit fixed-point rule, forward-mode tangent passes)",
+                "fp32_grad_evals_per_s": args.grad_seqs / legs["fp32"][0],
+                "fp64_grad_evals_per_s": args.grad_seqs / legs["fp64"][0],
+                "fp32_ms": legs["fp32"][0] * 1e3, "fp64_ms": legs["fp64"][0] * 1e3,
+                "fp32_over_fp64": legs["fp64"][0] / legs["fp32"][0],
+                "fp32_smoothed_step_its_per_s": nst * args.grad_iter / legs["fp32"][0],
+                "fp32_grad_rel_err_vs_fp64": float(np.abs(g32 - g64).max() / np.abs(g64).max()),
+                "fp32_ell_rel_err_vs_fp64": float(np.abs(legs["fp32"][1].sum() - legs["fp64"][1].sum())
+                                                  / abs(legs["fp64"][1].sum())),
+                "grad_fp64": g64.tolist()}
 
     value = T * n_iter / dt64
     # dominant kernel by total time
@@ -231,6 +301,7 @@ def main():
             "note": "FP64 VALU work (no MFMA on <=10x10 blocks); peak = MI355X FP64 vector rate"}
     cpu = None
     if not args.no_cpu:
+        log("CPU baseline")
         v, dtc, desc = cpu_baseline(min(args.cpu_sample, T), ys, n_iter, args.cpu_repeats)
         cpu = {"value": v, "unit": "step-it/s", "cores": 1, "kind": "port",
                "sample": f"{desc}, {dtc:.1f} s"}
@@ -247,6 +318,7 @@ def main():
         "alg_bytes_per_step_it": ALG_BYTES_WORDS * 8, "alg_flops_per_step_it": ALG_FLOPS,
         "kernel_ms": {k: v[0] / v[1] for k, v in rep.items()},
         "batch_fp32_cubature": batch,
+        "gradient": grad,
         "roofline": roof,
         "cpu_baseline": cpu,
     }

@@ -30,6 +30,14 @@ extern "C" {
 /* dtypes */
 #define PSM_F32 0
 #define PSM_F64 1
+/* Tangent (forward-mode) dtypes, square-root form only: every array element is
+ * a (value, tangent) pair of float / double (so a (n, nx) trajectory is stored
+ * as (n, nx, 2)); the log-likelihood output is a (value, tangent) pair of
+ * doubles per sequence.  Passing tangents of the inputs (model parameters and
+ * noise, prior, observations, nominal trajectory) yields the directional
+ * derivative of every output along them. */
+#define PSM_F32_TANGENT 2
+#define PSM_F64_TANGENT 3
 
 /* forms: MVNSqrt (chol factors) or MVNStandard (covariances) */
 #define PSM_FORM_SQRT 0
@@ -90,7 +98,7 @@ typedef struct psm_problem {
    * disables the check.  fallback_count (device, may be NULL) is incremented
    * once per recomputation; the caller zeroes it. */
   float fallback_ratio;
-  int32_t pad_;
+  int32_t scan_k;        /* scan items folded per lane (0: PSM_SCAN_K or 4) */
   unsigned long long* fallback_count;
 } psm_problem_t;
 
@@ -184,6 +192,22 @@ int psm_filter_smoother(const psm_problem_t* prob, const psm_model_t* trans,
  * The default stopping criterion of parsmooth/methods.py:50-51. */
 int psm_mean_sq_diff(int32_t dtype, int64_t n, const void* a, const void* b, double* out,
                      void* stream);
+
+/* Schedule autotuner (no reference counterpart: the reference's schedule is
+ * XLA's).  Times psm_filter_smoother (eager, on `stream`) for every candidate
+ * schedule -- chunk in {2, 4, 8, 16, 32} states per lane, scan fan-in
+ * scan_k in {2, 4, 8} -- whose workspace fits workspace_bytes, and returns the
+ * fastest in *best_chunk / *best_scan_k (and its time per pass in *best_ms,
+ * may be NULL).  prob->chunk / prob->scan_k are ignored.  The outputs
+ * (out_m, out_L, filt_m, filt_L) are overwritten; FP32 fallback counts
+ * accumulate.  Synchronizes the stream.  Results are cached per (dtype, form,
+ * nx, ny, n_seq, n_states, linearization) behind a mutex: a shape tuned before
+ * returns without running. */
+int psm_autotune(const psm_problem_t* prob, const psm_model_t* trans, const psm_model_t* obs,
+                 const psm_sigma_t* sigma, const void* ys, const void* m0, const void* L0,
+                 const void* nom_m, const void* nom_L, void* filt_m, void* filt_L, void* out_m,
+                 void* out_L, void* workspace, size_t workspace_bytes, void* stream,
+                 int32_t* best_chunk, int32_t* best_scan_k, float* best_ms);
 
 /* Drop every cached hipGraph (call before freeing buffers they reference). */
 void psm_clear_graphs(void);

@@ -16,6 +16,7 @@
 
 #define PSM_DECL(T, NX, NY) const psm::OpsTable* psm_ops_##T##_##NX##_##NY();
 PSM_FOR_ALL(PSM_DECL)
+PSM_FOR_DUAL(PSM_DECL)
 #undef PSM_DECL
 
 namespace psm {
@@ -25,17 +26,20 @@ void set_error(const std::string& msg) { g_err = msg; }
 
 static const OpsTable* lookup(int dtype, int form, int nx, int ny) {
   if (nx < 1 || nx > PSM_MAX_NX || ny < 1 || ny > PSM_MAX_NY) return nullptr;
-  if (dtype != PSM_F32 && dtype != PSM_F64) return nullptr;
+  if (dtype < PSM_F32 || dtype > PSM_F64_TANGENT) return nullptr;
   if (form != PSM_FORM_SQRT && form != PSM_FORM_STD) return nullptr;
-  static const OpsTable* table[2][2][PSM_MAX_NX + 1][PSM_MAX_NY + 1] = {};
+  static const OpsTable* table[4][2][PSM_MAX_NX + 1][PSM_MAX_NY + 1] = {};
   static std::once_flag once;
   std::call_once(once, [] {
 #define PSM_FILL(T, NX, NY) fill_##T(NX, NY, psm_ops_##T##_##NX##_##NY());
-    auto fill_f32s = [](int nx, int ny, const OpsTable* t) { table[0][0][nx][ny] = t; };
-    auto fill_f32c = [](int nx, int ny, const OpsTable* t) { table[0][1][nx][ny] = t; };
-    auto fill_f64s = [](int nx, int ny, const OpsTable* t) { table[1][0][nx][ny] = t; };
-    auto fill_f64c = [](int nx, int ny, const OpsTable* t) { table[1][1][nx][ny] = t; };
+    auto fill_f32s = [](int nx, int ny, const OpsTable* t) { table[PSM_F32][0][nx][ny] = t; };
+    auto fill_f32c = [](int nx, int ny, const OpsTable* t) { table[PSM_F32][1][nx][ny] = t; };
+    auto fill_f64s = [](int nx, int ny, const OpsTable* t) { table[PSM_F64][0][nx][ny] = t; };
+    auto fill_f64c = [](int nx, int ny, const OpsTable* t) { table[PSM_F64][1][nx][ny] = t; };
+    auto fill_d32s = [](int nx, int ny, const OpsTable* t) { table[PSM_F32_TANGENT][0][nx][ny] = t; };
+    auto fill_d64s = [](int nx, int ny, const OpsTable* t) { table[PSM_F64_TANGENT][0][nx][ny] = t; };
     PSM_FOR_ALL(PSM_FILL)
+    PSM_FOR_DUAL(PSM_FILL)
 #undef PSM_FILL
   });
   return table[dtype][form][nx][ny];
@@ -284,6 +288,7 @@ int psm_filter_smoother(const psm_problem_t* p, const psm_model_t* trans, const 
   std::vector<uint64_t> key;
   key.push_back(((uint64_t)p->dtype << 48) | ((uint64_t)p->form << 40) | ((uint64_t)p->nx << 32) |
                 ((uint64_t)p->ny << 16) | (uint64_t)(uint16_t)p->chunk);
+  key.push_back((uint64_t)(uint32_t)p->scan_k);
   key.push_back((uint64_t)p->n_seq);
   key.push_back((uint64_t)p->n_states);
   push(key, p->state_offsets);
@@ -405,6 +410,90 @@ int psm_timing_report(char* buf, size_t len) {
     buf[n] = 0;
   }
   return (int)out.size();
+}
+
+// ---------------------------------------------------------------- autotuner
+struct TuneResult {
+  int32_t chunk, scan_k;
+  float ms;
+};
+static std::mutex g_tune_mu;
+static std::map<std::vector<uint64_t>, TuneResult> g_tuned;
+
+int psm_autotune(const psm_problem_t* p, const psm_model_t* trans, const psm_model_t* obs,
+                 const psm_sigma_t* sigma, const void* ys, const void* m0, const void* L0,
+                 const void* nom_m, const void* nom_L, void* filt_m, void* filt_L, void* out_m,
+                 void* out_L, void* ws, size_t wsb, void* stream, int32_t* best_chunk,
+                 int32_t* best_scan_k, float* best_ms) {
+  int rc = check_problem(p);
+  if (rc) return rc;
+  if (best_chunk == nullptr || best_scan_k == nullptr) {
+    set_error("psm_autotune: null output pointer");
+    return PSM_E_ARG;
+  }
+  const OpsTable* t = need(p->dtype, p->form, p->nx, p->ny);
+  if (!t) return PSM_E_UNSUPPORTED;
+  std::vector<uint64_t> key = {(uint64_t)p->dtype, (uint64_t)p->form, (uint64_t)p->nx,
+                               (uint64_t)p->ny, (uint64_t)p->n_seq, (uint64_t)p->n_states,
+                               (uint64_t)(trans ? trans->kind : -1), (uint64_t)(obs ? obs->kind : -1),
+                               (uint64_t)(sigma ? sigma->npts : 0)};
+  std::lock_guard<std::mutex> lock(g_tune_mu);
+  auto hit = g_tuned.find(key);
+  if (hit != g_tuned.end()) {
+    *best_chunk = hit->second.chunk;
+    *best_scan_k = hit->second.scan_k;
+    if (best_ms) *best_ms = hit->second.ms;
+    return PSM_OK;
+  }
+  hipStream_t st = (hipStream_t)stream;
+  hipEvent_t e0, e1;
+  if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) {
+    set_error("psm_autotune: hipEventCreate failed");
+    return PSM_E_HIP;
+  }
+  TuneResult best{0, 0, 0.0f};
+  static const int32_t chunks[] = {2, 4, 8, 16, 32};
+  static const int32_t ks[] = {2, 4, 8};
+  constexpr int kReps = 2;
+  for (int32_t c : chunks) {
+    for (int32_t k : ks) {
+      psm_problem_t q = *p;
+      q.chunk = c;
+      q.scan_k = k;
+      if (t->workspace_bytes(&q) > wsb) continue;
+      rc = psm_filter_smoother(&q, trans, obs, sigma, ys, m0, L0, nom_m, nom_L, filt_m, filt_L,
+                               out_m, out_L, ws, wsb, 0, stream);
+      if (rc) break;
+      (void)hipEventRecord(e0, st);
+      for (int r = 0; r < kReps && rc == 0; ++r)
+        rc = psm_filter_smoother(&q, trans, obs, sigma, ys, m0, L0, nom_m, nom_L, filt_m, filt_L,
+                                 out_m, out_L, ws, wsb, 0, stream);
+      (void)hipEventRecord(e1, st);
+      if (rc) break;
+      if (hipEventSynchronize(e1) != hipSuccess) {
+        set_error("psm_autotune: hipEventSynchronize failed");
+        rc = PSM_E_HIP;
+        break;
+      }
+      float ms = 0.0f;
+      (void)hipEventElapsedTime(&ms, e0, e1);
+      ms /= kReps;
+      if (best.chunk == 0 || ms < best.ms) best = TuneResult{c, k, ms};
+    }
+    if (rc) break;
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (rc) return rc;
+  if (best.chunk == 0) {
+    set_error("psm_autotune: no candidate schedule fits the workspace");
+    return PSM_E_WORKSPACE;
+  }
+  g_tuned[key] = best;
+  *best_chunk = best.chunk;
+  *best_scan_k = best.scan_k;
+  if (best_ms) *best_ms = best.ms;
+  return PSM_OK;
 }
 
 void psm_clear_graphs(void) {

@@ -22,7 +22,6 @@ namespace psm {
 
 enum { FLAG_NORMAL = 0, FLAG_HEAD = 1, FLAG_IDENT = 2 };
 
-template <typename R> PSM_DEV double to_d(R x) { return (double)x; }
 
 // ----------------------------------------------------------------------------
 // linearized step (transition at state k-1, observation at state k)
@@ -180,7 +179,7 @@ PSM_DEV void copy_agg(const SAgg<R, NX>& s, SAgg<R, NX>& d) {
 // the innovation factor, which is what parsmooth/parallel/_filtering.py:149-154
 // computes from the filtered state of the previous step.
 template <typename R, int NX, int NY>
-PSM_DEV double sqrt_kf_step(const Step<R, NX, NY>& s, const R (&y)[NY], R (&m)[NX], R (&L)[NX][NX]) {
+PSM_DEV typename AccOf<R>::type sqrt_kf_step(const Step<R, NX, NY>& s, const R (&y)[NY], R (&m)[NX], R (&L)[NX][NX]) {
   R m1[NX];
   mv(s.F, m, m1);
 #pragma unroll
@@ -220,7 +219,8 @@ PSM_DEV double sqrt_kf_step(const Step<R, NX, NY>& s, const R (&y)[NY], R (&m)[N
   tria_inplace<R, NY + NX, NX + NY>(P);
   // innovation v = y - H m1 - c ; w = P11^{-1} v
   R w[NY];
-  double quad = 0.0, logdet = 0.0;
+  using Acc = typename AccOf<R>::type;
+  Acc quad = Acc(0.0), logdet = Acc(0.0);
 #pragma unroll
   for (int i = 0; i < NY; ++i) {
     R v = y[i] - s.c[i];
@@ -229,8 +229,8 @@ PSM_DEV double sqrt_kf_step(const Step<R, NX, NY>& s, const R (&y)[NY], R (&m)[N
 #pragma unroll
     for (int j = 0; j < i; ++j) v = fma(-P[i][j], w[j], v);
     w[i] = v / P[i][i];
-    quad += to_d(w[i]) * to_d(w[i]);
-    logdet += log(fabs(to_d(P[i][i])));
+    quad += to_acc(w[i]) * to_acc(w[i]);
+    logdet += log_(abs_(to_acc(P[i][i])));
   }
 #pragma unroll
   for (int i = 0; i < NX; ++i) {
@@ -241,7 +241,7 @@ PSM_DEV double sqrt_kf_step(const Step<R, NX, NY>& s, const R (&y)[NY], R (&m)[N
 #pragma unroll
     for (int j = 0; j < NX; ++j) L[i][j] = P[NY + i][NY + j];
   }
-  return -0.5 * quad - logdet - 0.5 * NY * 1.8378770664093454836;  // log(2 pi)
+  return Acc(-0.5) * quad - logdet - Acc(0.5 * NY * 1.8378770664093454836);  // log(2 pi)
 }
 
 // Absorb one more step into a filter aggregate: agg <- agg (+) element(step).
@@ -706,7 +706,7 @@ PSM_DEV void solve_lower_T_v(const R (&L)[N][N], const R (&b)[N], R (&x)[N]) {
 // log-likelihood from the Cholesky factor of S as in
 // parsmooth/parallel/_filtering.py:157-163.
 template <typename R, int NX, int NY>
-PSM_DEV double std_kf_step(const Step<R, NX, NY>& s, const R (&y)[NY], R (&m)[NX], R (&P)[NX][NX]) {
+PSM_DEV typename AccOf<R>::type std_kf_step(const Step<R, NX, NY>& s, const R (&y)[NY], R (&m)[NX], R (&P)[NX][NX]) {
   R m1[NX], FP[NX][NX], P1[NX][NX];
   mv(s.F, m, m1);
 #pragma unroll
@@ -736,11 +736,12 @@ PSM_DEV double std_kf_step(const Step<R, NX, NY>& s, const R (&y)[NY], R (&m)[NX
   }
   solve_lower_v(LS, v, w);
   solve_lower_T_v(LS, w, u);
-  double quad = 0.0, logdet = 0.0;
+  using Acc = typename AccOf<R>::type;
+  Acc quad = Acc(0.0), logdet = Acc(0.0);
 #pragma unroll
   for (int i = 0; i < NY; ++i) {
-    quad += to_d(w[i]) * to_d(w[i]);
-    logdet += log(fabs(to_d(LS[i][i])));
+    quad += to_acc(w[i]) * to_acc(w[i]);
+    logdet += log_(abs_(to_acc(LS[i][i])));
   }
   // KT = S^{-1} H P1 ; m = m1 + (H P1)^T S^{-1} v ; P = P1 - (H P1)^T S^{-1} (H P1)
   R T1[NY][NX], KT[NY][NX];
@@ -760,7 +761,7 @@ PSM_DEV double std_kf_step(const Step<R, NX, NY>& s, const R (&y)[NY], R (&m)[NX
       P[i][j] = a;
     }
   }
-  return -0.5 * quad - logdet - 0.5 * NY * 1.8378770664093454836;
+  return Acc(-0.5) * quad - logdet - Acc(0.5 * NY * 1.8378770664093454836);
 }
 
 // Covariance-form absorb of one step into (A, b, C, eta, J).
@@ -1046,7 +1047,7 @@ PSM_DEV void std_smoother_apply(const SAgg<R, NX>& e, R (&m)[NX], R (&Ps)[NX][NX
 // form dispatch + segmented (flagged) combines
 // ============================================================================
 template <typename R, int NX, int NY, bool SQRT>
-PSM_DEV double kf_step(const Step<R, NX, NY>& s, const R (&y)[NY], R (&m)[NX], R (&L)[NX][NX]) {
+PSM_DEV typename AccOf<R>::type kf_step(const Step<R, NX, NY>& s, const R (&y)[NY], R (&m)[NX], R (&L)[NX][NX]) {
   if constexpr (SQRT) return sqrt_kf_step<R, NX, NY>(s, y, m, L);
   else return std_kf_step<R, NX, NY>(s, y, m, L);
 }
@@ -1167,7 +1168,7 @@ PSM_DEV void cvt_agg(const SAgg<double, NX>& a, SAgg<float, NX>& b) {
 }
 
 template <typename R, int NX, int NY, bool SQRT>
-PSM_DEV double kf_step_fb(const Step<R, NX, NY>& s, const R (&y)[NY], R (&m)[NX], R (&L)[NX][NX],
+PSM_DEV typename AccOf<R>::type kf_step_fb(const Step<R, NX, NY>& s, const R (&y)[NY], R (&m)[NX], R (&L)[NX][NX],
                           const Fallback& fb) {
   if constexpr (kFallback<R, SQRT>) {
     float m0[NX], L0[NX][NX];

@@ -40,13 +40,16 @@ struct ScanLevel {
   int64_t n;      // items at this level
   int K;          // items per lane
   int64_t nwg;    // workgroups
+  int64_t stride; // padded item count (component stride of the interleaved layout)
   size_t off_in;  // byte offsets in the workspace
   size_t off_out;
 };
 
 struct Plan {
   int64_t M = 0;
+  int64_t Mp = 0;  // padded state count (component stride of the per-state buffers)
   int chunk = 0;
+  int wg = 0;      // scan workgroup size
   int64_t nchunks = 0;
   int nlevels = 0;
   ScanLevel lv[16];
@@ -68,37 +71,42 @@ inline int default_chunk(int64_t M) {
   return 4;
 }
 
-inline Plan make_plan(const psm_problem_t* p, size_t rsize, int nlin, int nqf, int nqs) {
+inline Plan make_plan(const psm_problem_t* p, size_t rsize, int nlin, int nqf, int nqs,
+                      size_t accsize = sizeof(double), int wg = kScanWG) {
   Plan pl;
   pl.M = p->n_states;
   pl.chunk = p->chunk > 0 ? p->chunk : env_int("PSM_CHUNK", default_chunk(pl.M));
   pl.nchunks = (pl.M + pl.chunk - 1) / pl.chunk;
+  pl.Mp = state_stride(pl.M, pl.chunk);
+  pl.wg = wg;
   const int nq = nqf > nqs ? nqf : nqs;
   size_t off = 0;
   pl.off_lin = off;
-  off = align_up(off + (size_t)nlin * pl.M * rsize);
+  off = align_up(off + (size_t)nlin * pl.Mp * rsize);
   pl.off_ell = off;
-  off = align_up(off + (size_t)pl.M * sizeof(double));
+  off = align_up(off + (size_t)pl.M * accsize);
   pl.off_selem = off;
-  off = align_up(off + (size_t)nqs * pl.M * rsize);
-  const int kmax = env_int("PSM_SCAN_K", 4);
+  off = align_up(off + (size_t)nqs * pl.Mp * rsize);
+  const int kmax = p->scan_k > 0 ? p->scan_k : env_int("PSM_SCAN_K", 4);
   int64_t n = pl.nchunks;
   int l = 0;
   while (true) {
     ScanLevel& s = pl.lv[l];
     s.n = n;
-    if (n <= (int64_t)kScanWG * 16) {
-      s.K = (int)((n + kScanWG - 1) / kScanWG);
+    if (n <= (int64_t)wg * 16) {
+      s.K = (int)((n + wg - 1) / wg);
       if (s.K < 1) s.K = 1;
       s.nwg = 1;
     } else {
       s.K = kmax;
-      s.nwg = (n + (int64_t)kScanWG * kmax - 1) / ((int64_t)kScanWG * kmax);
+      s.nwg = (n + (int64_t)wg * kmax - 1) / ((int64_t)wg * kmax);
     }
+    const int64_t blk = (int64_t)wg * s.K;
+    s.stride = (n + blk - 1) / blk * blk;
     s.off_in = off;
-    off = align_up(off + (size_t)nq * n * rsize);
+    off = align_up(off + (size_t)nq * s.stride * rsize);
     s.off_out = off;
-    off = align_up(off + (size_t)nq * n * rsize);
+    off = align_up(off + (size_t)nq * s.stride * rsize);
     ++l;
     if (s.nwg == 1 || l == 16) break;
     n = s.nwg;
@@ -163,8 +171,17 @@ struct Impl {
   using SA = SAgg<R, NX>;
   static constexpr int NLIN = Step<R, NX, NY>::NQ;
 
+  using Acc = typename AccOf<R>::type;
+  // Square-root form in float / double: team-cooperative scan kernels (WG =
+  // teams per workgroup); otherwise one lane per combination (WG = lanes).
+  static constexpr bool TEAM = kTeamScan && SQRT &&
+                               (std::is_same<R, float>::value || std::is_same<R, double>::value);
+  static constexpr int WG =
+      TEAM ? kTeamVL
+           : (scan_wg<FA, R>() < scan_wg<SA, R>() ? scan_wg<FA, R>() : scan_wg<SA, R>());
+
   static Plan plan(const psm_problem_t* p) {
-    return make_plan(p, sizeof(R), NLIN, FA::NQ, SA::NQ);
+    return make_plan(p, sizeof(R), NLIN, FA::NQ, SA::NQ, sizeof(Acc), WG);
   }
 
   static size_t workspace_bytes(const psm_problem_t* p) { return plan(p).total; }
@@ -246,19 +263,34 @@ struct Impl {
 
   // Exclusive scan of the level-0 aggregates; *final_off receives the workspace
   // offset of the complete exclusive prefixes.
+  static LevelMap level_map(const Plan& pl, int l, int rev) {
+    const ScanLevel& s = pl.lv[l];
+    return LevelMap{s.n, s.stride, s.K, WG, rev};
+  }
+
+  // Exclusive scan of the level-0 items (chunk aggregates, already in scan
+  // order in the interleaved layout); the complete prefixes end up at
+  // *final_off, in the same layout.
   template <typename Agg>
-  static int run_scan(const Plan& pl, char* ws, int reverse, const Fallback& fb, hipStream_t st,
+  static int run_scan(const Plan& pl, char* ws, const Fallback& fb, hipStream_t st,
                       size_t* final_off) {
-    const bool filt = (Agg::NQ == FA::NQ) && !reverse;
+    constexpr bool filt = std::is_same<Agg, FA>::value;
     for (int l = 0; l < pl.nlevels; ++l) {
       const ScanLevel& s = pl.lv[l];
       R* in = (R*)(ws + s.off_in);
       R* out = (R*)(ws + s.off_out);
-      R* tot = (l + 1 < pl.nlevels) ? (R*)(ws + pl.lv[l + 1].off_in) : nullptr;
+      const bool top = l + 1 == pl.nlevels;
+      R* tot = top ? nullptr : (R*)(ws + pl.lv[l + 1].off_in);
+      const LevelMap tm = top ? LevelMap{0, 0, 1, WG, 0} : level_map(pl, l + 1, 0);
       TimedScope ts_(l == 0 ? (filt ? "k_fscan_up_l0" : "k_sscan_up_l0")
                             : (filt ? "k_fscan_up" : "k_sscan_up"), st);
-      hipLaunchKernelGGL((k_scan_up<Agg, R, SQRT>), dim3((unsigned)s.nwg), dim3(kScanWG), 0, st,
-                         (const R*)in, s.n, s.K, (l == 0) ? reverse : 0, out, tot, s.nwg, fb);
+      if constexpr (TEAM)
+        hipLaunchKernelGGL((k_scan_up_team<Agg, R>), dim3((unsigned)s.nwg),
+                           dim3(TeamCfg<Agg, R>::WGT), 0, st, (const R*)in, level_map(pl, l, 0), out,
+                           tot, tm, fb);
+      else
+        hipLaunchKernelGGL((k_scan_up<Agg, R, SQRT, WG>), dim3((unsigned)s.nwg), dim3(WG), 0,
+                           st, (const R*)in, level_map(pl, l, 0), out, tot, tm, fb);
       int rc = hip_check("k_scan_up");
       if (rc) return rc;
     }
@@ -268,9 +300,16 @@ struct Impl {
       const size_t upper_off =
           (l + 1 == pl.nlevels - 1) ? pl.lv[l + 1].off_out : pl.lv[l + 1].off_in;
       TimedScope ts_(filt ? "k_fscan_down" : "k_sscan_down", st);
-      hipLaunchKernelGGL((k_scan_down<Agg, R, SQRT>), dim3((unsigned)s.nwg), dim3(kScanWG), 0, st,
-                         (const R*)(ws + s.off_out), s.n, s.K, (l == 0) ? reverse : 0,
-                         (const R*)(ws + upper_off), pl.lv[l + 1].n, (R*)(ws + s.off_in), fb);
+      if constexpr (TEAM)
+        hipLaunchKernelGGL((k_scan_down_team<Agg, R>), dim3((unsigned)s.nwg),
+                           dim3(TeamCfg<Agg, R>::WGT), 0, st, (const R*)(ws + s.off_out),
+                           level_map(pl, l, 0), (const R*)(ws + upper_off), level_map(pl, l + 1, 0),
+                           (R*)(ws + s.off_in), fb);
+      else
+        hipLaunchKernelGGL((k_scan_down<Agg, R, SQRT, WG>), dim3((unsigned)s.nwg), dim3(WG), 0,
+                           st, (const R*)(ws + s.off_out), level_map(pl, l, 0),
+                           (const R*)(ws + upper_off), level_map(pl, l + 1, 0),
+                           (R*)(ws + s.off_in), fb);
       int rc = hip_check("k_scan_down");
       if (rc) return rc;
     }
@@ -285,8 +324,9 @@ struct Impl {
     SigmaView<R> sv = sigma_view<R>(sg, &use);
     ModelView<R> obv = ob ? model_view<R>(ob) : model_view<R>(tr);
     TimedScope ts_("k_linearize_states", st);
-    hipLaunchKernelGGL((k_linearize_states<R, NX, NY, SQRT>), dim3(grid_for(pl.M, kLaneWG)),
-                       dim3(kLaneWG), 0, st, pl.M, model_view<R>(tr), obv, ob ? 1 : 0, sv, use,
+    hipLaunchKernelGGL((k_linearize_states<R, NX, NY, SQRT>), dim3(grid_for(pl.Mp, kLaneWG)),
+                       dim3(kLaneWG), 0, st, pl.M, pl.Mp, pl.chunk, model_view<R>(tr), obv,
+                       ob ? 1 : 0, sv, use,
                        (const R*)nm, (const R*)nL, (R*)(ws + pl.off_lin));
     return hip_check("k_linearize_states");
   }
@@ -312,25 +352,26 @@ struct Impl {
     {
     TimedScope ts_("k_filter_reduce", st);
     hipLaunchKernelGGL((k_filter_reduce<R, NX, NY, SQRT>), dim3(grid_for(pl.nchunks, kLaneWG)),
-                       dim3(kLaneWG), 0, st, pl.M, pl.chunk, pl.nchunks, p->state_offsets,
+                       dim3(kLaneWG), 0, st, pl.M, pl.Mp, pl.chunk, pl.nchunks, p->state_offsets,
                        p->n_seq, lin, (const R*)ys, (const R*)m0, (const R*)L0,
-                       (R*)(ws + pl.lv[0].off_in), fb);
+                       (R*)(ws + pl.lv[0].off_in), level_map(pl, 0, 0), fb);
     }
     if ((rc = hip_check("k_filter_reduce"))) return rc;
     size_t pre_off = 0;
-    if ((rc = run_scan<FA>(pl, ws, 0, fb, st, &pre_off))) return rc;
-    double* ell_state = ell ? (double*)(ws + pl.off_ell) : nullptr;
+    if ((rc = run_scan<FA>(pl, ws, fb, st, &pre_off))) return rc;
+    Acc* ell_state = ell ? (Acc*)(ws + pl.off_ell) : nullptr;
     {
     TimedScope ts_("k_filter_apply", st);
     hipLaunchKernelGGL((k_filter_apply<R, NX, NY, SQRT>), dim3(grid_for(pl.nchunks, kLaneWG)),
-                       dim3(kLaneWG), 0, st, pl.M, pl.chunk, pl.nchunks, p->state_offsets,
+                       dim3(kLaneWG), 0, st, pl.M, pl.Mp, pl.chunk, pl.nchunks, p->state_offsets,
                        p->n_seq, lin, (const R*)ys, (const R*)m0, (const R*)L0,
-                       (const R*)(ws + pre_off), (R*)om, (R*)oL, ell_state, fb);
+                       (const R*)(ws + pre_off), level_map(pl, 0, 0), (R*)om, (R*)oL, ell_state,
+                       fb);
     }
     if ((rc = hip_check("k_filter_apply"))) return rc;
     if (ell) {
-      hipLaunchKernelGGL(k_ell_reduce<256>, dim3((unsigned)p->n_seq), dim3(256), 0, st,
-                         (const double*)ell_state, p->state_offsets, ell);
+      hipLaunchKernelGGL((k_ell_reduce<256, Acc>), dim3((unsigned)p->n_seq), dim3(256), 0, st,
+                         (const Acc*)ell_state, p->state_offsets, (Acc*)ell);
       if ((rc = hip_check("k_ell_reduce"))) return rc;
     }
     return PSM_OK;
@@ -355,19 +396,20 @@ struct Impl {
     {
     TimedScope ts_("k_smoother_reduce", st);
     hipLaunchKernelGGL((k_smoother_reduce<R, NX, NY, SQRT>), dim3(grid_for(pl.nchunks, kLaneWG)),
-                       dim3(kLaneWG), 0, st, pl.M, pl.chunk, pl.nchunks, p->state_offsets,
+                       dim3(kLaneWG), 0, st, pl.M, pl.Mp, pl.chunk, pl.nchunks, p->state_offsets,
                        p->n_seq, (const R*)(ws + pl.off_lin), (const R*)fm, (const R*)fL,
-                       (R*)(ws + pl.off_selem), (R*)(ws + pl.lv[0].off_in), fb);
+                       (R*)(ws + pl.off_selem), (R*)(ws + pl.lv[0].off_in), level_map(pl, 0, 1),
+                       fb);
     }
     if ((rc = hip_check("k_smoother_reduce"))) return rc;
     size_t pre_off = 0;
-    if ((rc = run_scan<SA>(pl, ws, 1, fb, st, &pre_off))) return rc;
+    if ((rc = run_scan<SA>(pl, ws, fb, st, &pre_off))) return rc;
     {
       TimedScope ts_("k_smoother_apply", st);
       hipLaunchKernelGGL((k_smoother_apply<R, NX, SQRT>), dim3(grid_for(pl.nchunks, kLaneWG)),
-                         dim3(kLaneWG), 0, st, pl.M, pl.chunk, pl.nchunks,
-                         (const R*)(ws + pl.off_selem), (const R*)(ws + pre_off), (R*)om,
-                         (R*)oL, fb);
+                         dim3(kLaneWG), 0, st, pl.M, pl.Mp, pl.chunk, pl.nchunks,
+                         (const R*)(ws + pl.off_selem), (const R*)(ws + pre_off),
+                         level_map(pl, 0, 1), (R*)om, (R*)oL, fb);
     }
     return hip_check("k_smoother_apply");
   }

@@ -6,9 +6,10 @@
 //  * Caller-visible trajectories are array-of-structs, exactly the shapes of the
 //    reference: mean (n_states, nx), chol/cov (n_states, nx, nx).
 //  * Internal per-state buffers (linearized model, smoother elements) and the
-//    per-chunk aggregates are struct-of-arrays: component q of item i at
-//    base[q * n_items + i], so a wave touching consecutive items reads
-//    consecutive addresses.
+//    scan items are struct-of-arrays, component q of an item at
+//    base[q * stride + position], with positions interleaved (StateMap,
+//    LevelMap) so that the items a wave works on in one step are consecutive
+//    words.
 //
 // One filter pass over a ragged batch (all sequences concatenated, n_states
 // states) is
@@ -35,11 +36,58 @@
 
 #include "psm_core.hpp"
 #include "psm_scan_mem.hpp"
+#include "psm_team.hpp"
 
 namespace psm {
 
 constexpr int kScanWG = 64;     // one wave per scan workgroup
+// Scan workgroup size for an aggregate type: one wave, or half a wave when two
+// LDS buffers of 64 aggregates would not fit the 160 KB of LDS (tangent types).
+template <typename Agg, typename R>
+constexpr int scan_wg() {
+  return (2 * Agg::NQ * 64 * (int)sizeof(R) <= 150 * 1024) ? 64 : 32;
+}
 constexpr int kLaneWG = 64;     // workgroup size of the lane-per-item kernels
+
+// Positions in the internal struct-of-arrays buffers.  The lane-per-chunk
+// kernels run 64 consecutive chunks in one wave and step through their C
+// states in lock step, so slot k of chunk c is stored at
+//   (c / 64) * 64 C + 64 k + c % 64,
+// and one component of one slot is 64 consecutive words for the wave (one
+// full-line load instead of a C-strided gather).
+struct StateMap {
+  int C;
+  PSM_DEV int64_t operator()(int64_t i) const {
+    const int64_t c = i / C;
+    const int64_t k = i - c * C;
+    return (c >> 6) * ((int64_t)C << 6) + (k << 6) + (c & 63);
+  }
+};
+inline int64_t state_stride(int64_t M, int C) {
+  const int64_t g = (int64_t)C * 64;
+  return (M + g - 1) / g * g;
+}
+
+// A scan level: n items in scan order, K per lane, WG lanes per workgroup.
+// Scan position p of lane l, fold step j (p = blk * WG K + l K + j) is stored at
+// blk * WG K + j WG + l, so every fold step of a workgroup touches WG
+// consecutive words.  Level 0 holds the chunk aggregates: the smoother scans
+// them in reverse (rev = 1), chunk c sitting at scan position n - 1 - c.
+struct LevelMap {
+  int64_t n;       // items
+  int64_t stride;  // padded item count (multiple of WG K): component stride
+  int K;
+  int WG;
+  int rev;
+  PSM_DEV int64_t at(int64_t p) const {
+    const int64_t blk = (int64_t)K * WG;
+    const int64_t b = p / blk;
+    const int64_t r = p - b * blk;
+    const int64_t l = r / K;
+    return b * blk + (r - l * K) * WG + l;
+  }
+  PSM_DEV int64_t pos(int64_t c) const { return at(rev ? n - 1 - c : c); }
+};
 
 PSM_DEV int seq_of(const int64_t* __restrict__ soff, int nseq, int64_t i) {
   int lo = 0, hi = nseq - 1;
@@ -99,30 +147,37 @@ PSM_DEV void linearize_point(const ModelView<R>& md, const SigmaView<R>& sp, int
   }
 }
 
+// Thread t fills position t of the state-interleaved buffer (coalesced stores);
+// its state is the inverse of StateMap.
 template <typename R, int NX, int NY, bool SQRT>
 __global__ __launch_bounds__(kLaneWG) void k_linearize_states(
-    int64_t M, ModelView<R> tr, ModelView<R> ob, int do_obs, SigmaView<R> sp, int use_sigma,
-    const R* __restrict__ nom_m, const R* __restrict__ nom_L, R* __restrict__ lin) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int64_t M, int64_t Mp, int C, ModelView<R> tr, ModelView<R> ob, int do_obs, SigmaView<R> sp,
+    int use_sigma, const R* __restrict__ nom_m, const R* __restrict__ nom_L, R* __restrict__ lin) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= Mp) return;
+  const int64_t g = (int64_t)C << 6;
+  const int64_t blk = t / g;
+  const int64_t r = t - blk * g;
+  const int64_t i = ((blk << 6) + (r & 63)) * C + (r >> 6);
   if (i >= M) return;
   R m[NX], L[NX][NX];
   ld_aos(nom_m, i, m);
   ld_aos(nom_L, i, L);
   {
-    R F[NX][NX], C[NX][NX], b[NX];
-    linearize_point<R, NX, NX, SQRT>(tr, sp, use_sigma, m, L, F, C, b);
+    R F[NX][NX], Cq[NX][NX], b[NX];
+    linearize_point<R, NX, NX, SQRT>(tr, sp, use_sigma, m, L, F, Cq, b);
     int q = 0;
-    st(lin, M, i, q, F);
-    st(lin, M, i, q, C);
-    st(lin, M, i, q, b);
+    st(lin, Mp, t, q, F);
+    st(lin, Mp, t, q, Cq);
+    st(lin, Mp, t, q, b);
   }
   if (do_obs) {
-    R H[NY][NX], C[NY][NY], c[NY];
-    linearize_point<R, NX, NY, SQRT>(ob, sp, use_sigma, m, L, H, C, c);
+    R H[NY][NX], Cr[NY][NY], c[NY];
+    linearize_point<R, NX, NY, SQRT>(ob, sp, use_sigma, m, L, H, Cr, c);
     int q = Step<R, NX, NY>::NTRANS;
-    st(lin, M, i, q, H);
-    st(lin, M, i, q, C);
-    st(lin, M, i, q, c);
+    st(lin, Mp, t, q, H);
+    st(lin, Mp, t, q, Cr);
+    st(lin, Mp, t, q, c);
   }
 }
 
@@ -131,9 +186,10 @@ __global__ __launch_bounds__(kLaneWG) void k_linearize_states(
 // ============================================================================
 template <typename R, int NX, int NY, bool SQRT>
 __global__ __launch_bounds__(kLaneWG) void k_filter_reduce(
-    int64_t M, int chunk, int64_t nchunks, const int64_t* __restrict__ soff, int nseq,
+    int64_t M, int64_t Mp, int chunk, int64_t nchunks, const int64_t* __restrict__ soff, int nseq,
     const R* __restrict__ lin, const R* __restrict__ ys, const R* __restrict__ m0,
-    const R* __restrict__ L0, R* __restrict__ agg, Fallback fb) {
+    const R* __restrict__ L0, R* __restrict__ agg, LevelMap lm, Fallback fb) {
+  const StateMap sm{chunk};
   const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= nchunks) return;
   const int64_t i0 = c * chunk;
@@ -152,59 +208,61 @@ __global__ __launch_bounds__(kLaneWG) void k_filter_reduce(
       a.flag = FLAG_HEAD;
     } else {
       Step<R, NX, NY> st_;
-      load_trans(lin, M, i - 1, st_);
-      load_obs(lin, M, i, st_);
+      load_trans(lin, Mp, sm(i - 1), st_);
+      load_obs(lin, Mp, sm(i), st_);
       R y[NY];
       ld_aos(ys, i - s - 1, y);
       absorb_fb<R, NX, NY, SQRT>(a, st_, y, fb);
     }
   }
-  store_agg(agg, nchunks, c, a);
+  store_agg(agg, lm.stride, lm.pos(c), a);
 }
 
 // ============================================================================
 // 3. exclusive scan of aggregates (filter: forward, smoother: reverse)
 // ============================================================================
-// in/out: n items (SoA, stride n).  Item index of scan position p is
-// (reverse ? n - 1 - p : p).  Workgroup w covers positions [w*WG*K, (w+1)*WG*K):
-// every lane folds its K items, a Kogge-Stone scan over the lanes runs in a
+// in/out: the items of one level (LevelMap: n items in scan order, stored
+// interleaved).  Workgroup w covers scan positions [w*WG*K, (w+1)*WG*K): every
+// lane folds its K items, a Kogge-Stone scan over the lanes runs in a
 // double-buffered LDS array, and the lane then writes the exclusive prefix of
-// each of its items (within the workgroup) to out_excl; totals[w] receives the
-// workgroup's inclusive total.  Aggregates live in LDS / HBM, never whole in
-// registers: the operator reads its operands from memory as it needs them.
-template <typename Agg, typename R, bool SQRT>
-__global__ __launch_bounds__(kScanWG) void k_scan_up(const R* __restrict__ in, int64_t n, int K,
-                                                     int reverse, R* __restrict__ out_excl,
-                                                     R* __restrict__ totals, int64_t ntot,
-                                                     Fallback fb) {
+// each of its items (within the workgroup) to out_excl; the workgroup's
+// inclusive total becomes item w of the next level (tm).  Aggregates live in
+// LDS / HBM, never whole in registers: the operator reads its operands from
+// memory as it needs them.
+template <typename Agg, typename R, bool SQRT, int WG = scan_wg<Agg, R>()>
+__global__ __launch_bounds__(WG) void k_scan_up(const R* __restrict__ in, LevelMap lm,
+                                                R* __restrict__ out_excl, R* __restrict__ totals,
+                                                LevelMap tm, Fallback fb) {
   using V = typename ViewOf<Agg>::type;
   constexpr int NX = Agg_nx<Agg>::value;
   constexpr int NQ = Agg::NQ;
-  __shared__ R lds[2 * NQ * kScanWG];
+  __shared__ R lds[2 * NQ * WG];
   const int t = threadIdx.x;
-  auto slot = [&](int buf, int lane) { return V{lds + buf * NQ * kScanWG, (int64_t)kScanWG, (int64_t)lane}; };
+  const int K = lm.K;
+  const int64_t n = lm.n;
+  auto slot = [&](int buf, int lane) { return V{lds + buf * NQ * WG, (int64_t)WG, (int64_t)lane}; };
   R* inm = const_cast<R*>(in);
-  const int64_t base = ((int64_t)blockIdx.x * kScanWG + t) * K;
+  const int64_t base = ((int64_t)blockIdx.x * WG + t) * K;
   int cur = 0;
   view_identity(slot(0, t));
   for (int j = 0; j < K; ++j) {
     const int64_t p = base + j;
     if (p >= n) break;
-    const V e{inm, n, reverse ? n - 1 - p : p};
+    const V e{inm, lm.stride, lm.at(p)};
     combine_mem_fb<R, NX, SQRT>(slot(cur, t), e, slot(1 - cur, t), fb);
     cur = 1 - cur;
   }
   if (cur == 1) view_copy(slot(1, t), slot(0, t));
   cur = 0;
-  for (int d = 1; d < kScanWG; d <<= 1) {
+  for (int d = 1; d < WG; d <<= 1) {
     __syncthreads();
     if (t >= d) combine_mem_fb<R, NX, SQRT>(slot(cur, t - d), slot(cur, t), slot(1 - cur, t), fb);
     else view_copy(slot(cur, t), slot(1 - cur, t));
     cur = 1 - cur;
   }
   __syncthreads();
-  if (t == kScanWG - 1 && totals != nullptr)
-    view_copy(slot(cur, t), V{totals, ntot, (int64_t)blockIdx.x});
+  if (t == WG - 1 && totals != nullptr)
+    view_copy(slot(cur, t), V{totals, tm.stride, tm.at((int64_t)blockIdx.x)});
   // running exclusive prefix in this lane's slot of the other buffer
   const V run = slot(1 - cur, t);
   if (t == 0) view_identity(run);
@@ -212,35 +270,137 @@ __global__ __launch_bounds__(kScanWG) void k_scan_up(const R* __restrict__ in, i
   for (int j = 0; j < K; ++j) {
     const int64_t p = base + j;
     if (p >= n) break;
-    const int64_t item = reverse ? n - 1 - p : p;
-    const V o{out_excl, n, item};
+    const int64_t item = lm.at(p);
+    const V o{out_excl, lm.stride, item};
     view_copy(run, o);
-    combine_mem_fb<R, NX, SQRT>(o, V{inm, n, item}, run, fb);
+    combine_mem_fb<R, NX, SQRT>(o, V{inm, lm.stride, item}, run, fb);
   }
 }
 
-// dst[item] <- upper[w] (+) out_excl[item] (workgroup w > 0; w == 0 copies):
-// the complete exclusive prefixes, written out of place so the operands stay
-// intact for the FP64 fallback.
-template <typename Agg, typename R, bool SQRT>
-__global__ __launch_bounds__(kScanWG) void k_scan_down(const R* __restrict__ out_excl, int64_t n,
-                                                       int K, int reverse,
-                                                       const R* __restrict__ upper,
-                                                       int64_t nupper, R* __restrict__ dst,
-                                                       Fallback fb) {
+// dst[p] <- upper[w] (+) out_excl[p] (workgroup w > 0; w == 0 copies): the
+// complete exclusive prefixes, written out of place so the operands stay
+// intact for the FP64 fallback.  upper holds the complete exclusive prefixes of
+// the next level (um).
+template <typename Agg, typename R, bool SQRT, int WG = scan_wg<Agg, R>()>
+__global__ __launch_bounds__(WG) void k_scan_down(const R* __restrict__ out_excl, LevelMap lm,
+                                                  const R* __restrict__ upper, LevelMap um,
+                                                  R* __restrict__ dst, Fallback fb) {
   using V = typename ViewOf<Agg>::type;
   constexpr int NX = Agg_nx<Agg>::value;
   const int t = threadIdx.x;
-  const int64_t base = ((int64_t)blockIdx.x * kScanWG + t) * K;
-  const V up{const_cast<R*>(upper), nupper, (int64_t)blockIdx.x};
+  const int K = lm.K;
+  const int64_t base = ((int64_t)blockIdx.x * WG + t) * K;
+  const V up{const_cast<R*>(upper), um.stride, um.at((int64_t)blockIdx.x)};
+  for (int j = 0; j < K; ++j) {
+    const int64_t p = base + j;
+    if (p >= lm.n) break;
+    const int64_t item = lm.at(p);
+    const V e{const_cast<R*>(out_excl), lm.stride, item};
+    const V o{dst, lm.stride, item};
+    if (blockIdx.x == 0) view_copy(e, o);
+    else combine_mem_fb<R, NX, SQRT>(up, e, o, fb);
+  }
+}
+
+// Team versions of the two scan kernels for the square-root form: kTeamVL
+// teams of TL lanes per workgroup; each team is one virtual lane of the scan
+// above (folds K items, joins the Kogge-Stone scan over the teams, rewrites
+// its items' exclusive prefixes) and runs every combination cooperatively.
+template <typename Agg, typename R>
+struct TeamCfg {
+  using V = typename ViewOf<Agg>::type;
+  static constexpr int TL = TeamOf<V>::TL;
+  static constexpr int VL = kTeamVL;
+  static constexpr int WGT = VL * TL;
+  static constexpr int SCR = team_scratch_bytes<V, R>();
+};
+
+template <typename Agg, typename R>
+__global__ __launch_bounds__((TeamCfg<Agg, R>::WGT)) void k_scan_up_team(
+    const R* __restrict__ in, LevelMap lm, R* __restrict__ out_excl, R* __restrict__ totals,
+    LevelMap tm, Fallback fb) {
+  using Cfg = TeamCfg<Agg, R>;
+  using V = typename Cfg::V;
+  constexpr int NX = Agg_nx<Agg>::value;
+  constexpr int NQ = Agg::NQ;
+  constexpr int TL = Cfg::TL;
+  constexpr int VL = Cfg::VL;
+  __shared__ R lds[2 * NQ * VL];
+  __shared__ __align__(16) char scr[VL * Cfg::SCR];
+  const int t = (int)threadIdx.x / TL;
+  const int r = (int)threadIdx.x % TL;
+  void* my = scr + t * Cfg::SCR;
+  auto slot = [&](int buf, int lane) { return V{lds + buf * NQ * VL, (int64_t)VL, (int64_t)lane}; };
+  const int K = lm.K;
+  const int64_t n = lm.n;
+  R* inm = const_cast<R*>(in);
+  const int64_t base = ((int64_t)blockIdx.x * VL + t) * K;
+  int cur = 0;
+  team_identity(slot(0, t), r);
   for (int j = 0; j < K; ++j) {
     const int64_t p = base + j;
     if (p >= n) break;
-    const int64_t item = reverse ? n - 1 - p : p;
-    const V e{const_cast<R*>(out_excl), n, item};
-    const V o{dst, n, item};
-    if (blockIdx.x == 0) view_copy(e, o);
-    else combine_mem_fb<R, NX, SQRT>(up, e, o, fb);
+    team_combine<R, NX>(slot(cur, t), V{inm, lm.stride, lm.at(p)}, slot(1 - cur, t), r, my, fb);
+    cur = 1 - cur;
+  }
+  if (cur == 1) {
+    team_sync();
+    team_copy(slot(1, t), slot(0, t), r);
+  }
+  cur = 0;
+  for (int d = 1; d < VL; d <<= 1) {
+    __syncthreads();
+    if (t >= d) team_combine<R, NX>(slot(cur, t - d), slot(cur, t), slot(1 - cur, t), r, my, fb);
+    else team_copy(slot(cur, t), slot(1 - cur, t), r);
+    cur = 1 - cur;
+  }
+  __syncthreads();
+  if (t == VL - 1 && totals != nullptr)
+    team_copy(slot(cur, t), V{totals, tm.stride, tm.at((int64_t)blockIdx.x)}, r);
+  // running exclusive prefix of this team: run = slot(1 - cur, t); once every
+  // team has read its predecessor's inclusive prefix, slot(cur, t) is free
+  V run = slot(1 - cur, t);
+  if (t == 0) team_identity(run, r);
+  else team_copy(slot(cur, t - 1), run, r);
+  __syncthreads();
+  V tmp = slot(cur, t);
+  for (int j = 0; j < K; ++j) {
+    const int64_t p = base + j;
+    if (p >= n) break;
+    const int64_t item = lm.at(p);
+    team_sync();
+    team_copy(run, V{out_excl, lm.stride, item}, r);
+    team_combine<R, NX>(run, V{inm, lm.stride, item}, tmp, r, my, fb);
+    const V x = run;
+    run = tmp;
+    tmp = x;
+  }
+}
+
+template <typename Agg, typename R>
+__global__ __launch_bounds__((TeamCfg<Agg, R>::WGT)) void k_scan_down_team(
+    const R* __restrict__ out_excl, LevelMap lm, const R* __restrict__ upper, LevelMap um,
+    R* __restrict__ dst, Fallback fb) {
+  using Cfg = TeamCfg<Agg, R>;
+  using V = typename Cfg::V;
+  constexpr int NX = Agg_nx<Agg>::value;
+  constexpr int TL = Cfg::TL;
+  constexpr int VL = Cfg::VL;
+  __shared__ __align__(16) char scr[VL * Cfg::SCR];
+  const int t = (int)threadIdx.x / TL;
+  const int r = (int)threadIdx.x % TL;
+  void* my = scr + t * Cfg::SCR;
+  const int K = lm.K;
+  const int64_t base = ((int64_t)blockIdx.x * VL + t) * K;
+  const V up{const_cast<R*>(upper), um.stride, um.at((int64_t)blockIdx.x)};
+  for (int j = 0; j < K; ++j) {
+    const int64_t p = base + j;
+    if (p >= lm.n) break;
+    const int64_t item = lm.at(p);
+    const V e{const_cast<R*>(out_excl), lm.stride, item};
+    const V o{dst, lm.stride, item};
+    if (blockIdx.x == 0) team_copy(e, o, r);
+    else team_combine<R, NX>(up, e, o, r, my, fb);
   }
 }
 
@@ -249,10 +409,12 @@ __global__ __launch_bounds__(kScanWG) void k_scan_down(const R* __restrict__ out
 // ============================================================================
 template <typename R, int NX, int NY, bool SQRT>
 __global__ __launch_bounds__(kLaneWG) void k_filter_apply(
-    int64_t M, int chunk, int64_t nchunks, const int64_t* __restrict__ soff, int nseq,
+    int64_t M, int64_t Mp, int chunk, int64_t nchunks, const int64_t* __restrict__ soff, int nseq,
     const R* __restrict__ lin, const R* __restrict__ ys, const R* __restrict__ m0,
-    const R* __restrict__ L0, const R* __restrict__ prefix, R* __restrict__ out_m,
-    R* __restrict__ out_L, double* __restrict__ ell_state, Fallback fb) {
+    const R* __restrict__ L0, const R* __restrict__ prefix, LevelMap lm, R* __restrict__ out_m,
+    R* __restrict__ out_L, typename AccOf<R>::type* __restrict__ ell_state, Fallback fb) {
+  using Acc = typename AccOf<R>::type;
+  const StateMap sm{chunk};
   const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= nchunks) return;
   const int64_t i0 = c * chunk;
@@ -262,19 +424,20 @@ __global__ __launch_bounds__(kLaneWG) void k_filter_apply(
   {
     // incoming filtered state: (b, U) of the exclusive prefix
     int q = NX * NX;
-    ld(prefix, nchunks, c, q, m);
-    ld(prefix, nchunks, c, q, L);
+    const int64_t pc = lm.pos(c);
+    ld(prefix, lm.stride, pc, q, m);
+    ld(prefix, lm.stride, pc, q, L);
   }
   for (int64_t i = i0; i < i1; ++i) {
     while (i >= soff[s + 1]) ++s;
-    double e = 0.0;
+    Acc e = Acc(0.0);
     if (i == soff[s]) {
       ld_aos(m0, s, m);
       ld_aos(L0, s, L);
     } else {
       Step<R, NX, NY> st_;
-      load_trans(lin, M, i - 1, st_);
-      load_obs(lin, M, i, st_);
+      load_trans(lin, Mp, sm(i - 1), st_);
+      load_obs(lin, Mp, sm(i), st_);
       R y[NY];
       ld_aos(ys, i - s - 1, y);
       e = kf_step_fb<R, NX, NY, SQRT>(st_, y, m, L, fb);
@@ -286,14 +449,14 @@ __global__ __launch_bounds__(kLaneWG) void k_filter_apply(
 }
 
 // per-sequence sum of the per-state log-likelihood terms (FP64)
-template <int BS>
-__global__ __launch_bounds__(BS) void k_ell_reduce(const double* __restrict__ ell_state,
+template <int BS, typename Acc>
+__global__ __launch_bounds__(BS) void k_ell_reduce(const Acc* __restrict__ ell_state,
                                                     const int64_t* __restrict__ soff,
-                                                    double* __restrict__ ell) {
-  __shared__ double red[BS];
+                                                    Acc* __restrict__ ell) {
+  __shared__ Acc red[BS];
   const int s = blockIdx.x;
   const int64_t a = soff[s], b = soff[s + 1];
-  double acc = 0.0;
+  Acc acc = Acc(0.0);
   for (int64_t i = a + threadIdx.x; i < b; i += blockDim.x) acc += ell_state[i];
   red[threadIdx.x] = acc;
   __syncthreads();
@@ -309,9 +472,10 @@ __global__ __launch_bounds__(BS) void k_ell_reduce(const double* __restrict__ el
 // ============================================================================
 template <typename R, int NX, int NY, bool SQRT>
 __global__ __launch_bounds__(kLaneWG) void k_smoother_reduce(
-    int64_t M, int chunk, int64_t nchunks, const int64_t* __restrict__ soff, int nseq,
+    int64_t M, int64_t Mp, int chunk, int64_t nchunks, const int64_t* __restrict__ soff, int nseq,
     const R* __restrict__ lin, const R* __restrict__ fm, const R* __restrict__ fL,
-    R* __restrict__ elems, R* __restrict__ agg, Fallback fb) {
+    R* __restrict__ elems, R* __restrict__ agg, LevelMap lm, Fallback fb) {
+  const StateMap sm{chunk};
   const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= nchunks) return;
   const int64_t i0 = c * chunk;
@@ -333,22 +497,25 @@ __global__ __launch_bounds__(kLaneWG) void k_smoother_reduce(
     } else {
       R F[NX][NX], cq[NX][NX], bt[NX];
       int q = 0;
-      ld(lin, M, i, q, F);
-      ld(lin, M, i, q, cq);
-      ld(lin, M, i, q, bt);
+      const int64_t pi = sm(i);
+      ld(lin, Mp, pi, q, F);
+      ld(lin, Mp, pi, q, cq);
+      ld(lin, Mp, pi, q, bt);
       smoother_element_fb<R, NX, SQRT>(F, cq, bt, m, L, e, fb);
     }
-    store_agg(elems, M, i, e);
+    store_agg(elems, Mp, sm(i), e);
     combine_into_fb<R, NX, SQRT>(acc, e, fb);
     copy_agg(e, acc);
   }
-  store_agg(agg, nchunks, c, acc);
+  store_agg(agg, lm.stride, lm.pos(c), acc);
 }
 
 template <typename R, int NX, bool SQRT>
 __global__ __launch_bounds__(kLaneWG) void k_smoother_apply(
-    int64_t M, int chunk, int64_t nchunks, const R* __restrict__ elems,
-    const R* __restrict__ prefix, R* __restrict__ out_m, R* __restrict__ out_L, Fallback fb) {
+    int64_t M, int64_t Mp, int chunk, int64_t nchunks, const R* __restrict__ elems,
+    const R* __restrict__ prefix, LevelMap lm, R* __restrict__ out_m, R* __restrict__ out_L,
+    Fallback fb) {
+  const StateMap sm{chunk};
   const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= nchunks) return;
   const int64_t i0 = c * chunk;
@@ -356,13 +523,14 @@ __global__ __launch_bounds__(kLaneWG) void k_smoother_apply(
   R m[NX], D[NX][NX];
   {
     int q = 0;
-    ld(prefix, nchunks, c, q, m);
+    const int64_t pc = lm.pos(c);
+    ld(prefix, lm.stride, pc, q, m);
     q += NX * NX;
-    ld(prefix, nchunks, c, q, D);
+    ld(prefix, lm.stride, pc, q, D);
   }
   for (int64_t i = i1 - 1; i >= i0; --i) {
     SAgg<R, NX> e;
-    load_agg(elems, M, i, e);
+    load_agg(elems, Mp, sm(i), e);
     if (e.flag == FLAG_HEAD) {
       copy(e.g, m);
       copy(e.D, D);

@@ -7,6 +7,7 @@
 // tests/test_linearization.py:157), each with an analytic Jacobian.
 #pragma once
 #include "psm_linalg.hpp"
+#include "psm_dual.hpp"
 #include "psm.h"
 
 namespace psm {
@@ -20,12 +21,8 @@ struct ModelView {
   const R* q_chol;   // noise chol (sqrt form) or covariance (covariance form), nout x nout
 };
 
-template <typename R>
-PSM_DEV R atan2_(R y, R x);
-template <>
-PSM_DEV double atan2_<double>(double y, double x) { return atan2(y, x); }
-template <>
-PSM_DEV float atan2_<float>(float y, float x) { return atan2f(y, x); }
+PSM_DEV double atan2_(double y, double x) { return atan2(y, x); }
+PSM_DEV float atan2_(float y, float x) { return atan2f(y, x); }
 PSM_DEV double sin_(double x) { return sin(x); }
 PSM_DEV float sin_(float x) { return sinf(x); }
 PSM_DEV double cos_(double x) { return cos(x); }
@@ -34,6 +31,19 @@ PSM_DEV double exp_(double x) { return exp(x); }
 PSM_DEV float exp_(float x) { return expf(x); }
 PSM_DEV double atan_(double x) { return atan(x); }
 PSM_DEV float atan_(float x) { return atanf(x); }
+
+// tangent rules of the model functions
+template <typename B> PSM_DEV DualT<B> sin_(DualT<B> a) { return {sin_(a.v), cos_(a.v) * a.d}; }
+template <typename B> PSM_DEV DualT<B> cos_(DualT<B> a) { return {cos_(a.v), -sin_(a.v) * a.d}; }
+template <typename B> PSM_DEV DualT<B> exp_(DualT<B> a) {
+  const B e = exp_(a.v);
+  return {e, e * a.d};
+}
+template <typename B> PSM_DEV DualT<B> atan_(DualT<B> a) { return {atan_(a.v), a.d / (B(1) + a.v * a.v)}; }
+template <typename B> PSM_DEV DualT<B> atan2_(DualT<B> y, DualT<B> x) {
+  const B q = x.v * x.v + y.v * y.v;
+  return {atan2_(y.v, x.v), (x.v * y.d - y.v * x.d) / q};
+}
 
 // f(x) and, when JAC, its Jacobian J (NOUT x NIN).
 template <typename R, int NIN, int NOUT, bool JAC>

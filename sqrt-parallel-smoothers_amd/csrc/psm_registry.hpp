@@ -7,3 +7,6 @@
   PSM_FOR_TAG(X, f32s) PSM_FOR_TAG(X, f32c) PSM_FOR_TAG(X, f64s) PSM_FOR_TAG(X, f64c)
 #define PSM_MAX_NX 5
 #define PSM_MAX_NY 3
+
+// Tangent (forward-mode) slices, square-root form only.
+#define PSM_FOR_DUAL(X) X(d64s, 1, 1) X(d64s, 2, 1) X(d64s, 2, 2) X(d64s, 3, 2) X(d64s, 4, 2) X(d64s, 5, 2) X(d32s, 5, 2)

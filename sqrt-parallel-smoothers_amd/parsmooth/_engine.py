@@ -79,6 +79,42 @@ class Problem:
             N.check(-1, "psm_workspace_bytes")
         self._ws = None
 
+    # candidate schedules of the library's autotuner (chunk, scan fan-in)
+    TUNE_CHUNKS = (2, 4, 8, 16, 32)
+    TUNE_SCAN_K = (2, 4, 8)
+
+    def set_schedule(self, chunk: int, scan_k: int):
+        """Fix the launch schedule (0 = library default) and resize the workspace."""
+        self.struct.chunk = int(chunk)
+        self.struct.scan_k = int(scan_k)
+        nbytes = int(N.lib().psm_workspace_bytes(ctypes.byref(self.struct)))
+        if nbytes == 0:
+            N.check(-1, "psm_workspace_bytes")
+        if self._ws is not None and self._ws.numel() < nbytes:
+            self._ws = None
+        self.ws_bytes = nbytes
+
+    def autotune(self, mods, ys, m0, L0, nom_m, nom_L, filt, out):
+        """Time every candidate schedule of one filter-smoother pass at the given
+        nominal and keep the fastest; returns (chunk, scan_k, ms per pass)."""
+        saved = (self.struct.chunk, self.struct.scan_k)
+        need = 0
+        for c in self.TUNE_CHUNKS:
+            for k in self.TUNE_SCAN_K:
+                self.struct.chunk, self.struct.scan_k = c, k
+                need = max(need, int(N.lib().psm_workspace_bytes(ctypes.byref(self.struct))))
+        self.struct.chunk, self.struct.scan_k = saved
+        ws = torch.empty(need, dtype=torch.uint8, device=self.dev)
+        bc, bk, ms = ctypes.c_int32(0), ctypes.c_int32(0), ctypes.c_float(0.0)
+        rc = N.lib().psm_autotune(ctypes.byref(self.struct), ctypes.byref(mods.trans), ctypes.byref(mods.obs),
+                                  mods.sigma_ref(), N.ptr(ys), N.ptr(m0), N.ptr(L0), N.ptr(nom_m), N.ptr(nom_L),
+                                  N.ptr(filt[0]), N.ptr(filt[1]), N.ptr(out[0]), N.ptr(out[1]), N.ptr(ws),
+                                  need, N.stream_ptr(), ctypes.byref(bc), ctypes.byref(bk), ctypes.byref(ms))
+        N.check(rc, "psm_autotune")
+        del ws
+        self.set_schedule(bc.value, bk.value)
+        return bc.value, bk.value, ms.value
+
     @property
     def workspace(self) -> torch.Tensor:
         if self._ws is None:

@@ -33,7 +33,7 @@ EXPORTS = (
     "psm_version", "psm_last_error", "psm_supported", "psm_linearize", "psm_filter_elements",
     "psm_filter_combine", "psm_smoother_elements", "psm_smoother_combine", "psm_workspace_bytes",
     "psm_filter", "psm_smoother", "psm_filter_smoother", "psm_mean_sq_diff", "psm_clear_graphs",
-    "psm_timing_enable", "psm_timing_report", "psm_chunk_size",
+    "psm_timing_enable", "psm_timing_report", "psm_chunk_size", "psm_autotune",
 )
 
 
@@ -52,7 +52,7 @@ class PsmProblem(ctypes.Structure):
     _fields_ = [("dtype", ctypes.c_int32), ("form", ctypes.c_int32), ("nx", ctypes.c_int32),
                 ("ny", ctypes.c_int32), ("n_seq", ctypes.c_int32), ("chunk", ctypes.c_int32),
                 ("n_states", ctypes.c_int64), ("state_offsets", ctypes.c_void_p),
-                ("fallback_ratio", ctypes.c_float), ("pad_", ctypes.c_int32),
+                ("fallback_ratio", ctypes.c_float), ("scan_k", ctypes.c_int32),
                 ("fallback_count", ctypes.c_void_p)]
 
 
@@ -86,6 +86,10 @@ _SIGS = {
     "psm_timing_enable": (None, [_I32]),
     "psm_chunk_size": (ctypes.c_int, [ctypes.POINTER(PsmProblem)]),
     "psm_timing_report": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_size_t]),
+    "psm_autotune": (ctypes.c_int, [ctypes.POINTER(PsmProblem), ctypes.POINTER(PsmModel),
+                                    ctypes.POINTER(PsmModel), ctypes.POINTER(PsmSigma)]
+                     + [_P] * 10 + [ctypes.c_size_t, _P, ctypes.POINTER(_I32), ctypes.POINTER(_I32),
+                                    ctypes.POINTER(ctypes.c_float)]),
 }
 
 

@@ -54,6 +54,23 @@ class IteratedSmoother:
         """Number of FP64 recomputations since construction (FP32 square-root form)."""
         return int(self.prob.fallback_counter.item())
 
+    def autotune(self, nominal=None):
+        """Pick the fastest launch schedule (chunk, scan fan-in) for this batch
+        by timing one filter-smoother pass per candidate at ``nominal`` (the
+        default nominal when None); returns (chunk, scan_k, ms per pass).  Cached
+        graphs are dropped, since the schedule changes the launch sequence."""
+        if self.ys is None:
+            raise RuntimeError("call set_data first")
+        if nominal is None:
+            nominal = self._default
+        else:
+            nominal = (to_dev(nominal[0], self.dtype, self.dev).reshape(self._init[0].shape),
+                       to_dev(nominal[1], self.dtype, self.dev).reshape(self._init[1].shape))
+        res = self.prob.autotune(self.mods, self.ys, self.m0, self.L0, nominal[0], nominal[1], self._filt,
+                                 self._bufs[1])
+        N.lib().psm_clear_graphs()
+        return res
+
     def chunk_size(self) -> int:
         """States per lane in the chunk passes (as resolved by the library)."""
         import ctypes

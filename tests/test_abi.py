@@ -37,6 +37,22 @@ def test_supported_slices():
                     assert lib.psm_supported(dtype, form, nx, ny) == 1
     assert lib.psm_supported(N.PSM_F64, N.FORM_SQRT, 6, 2) == 0
     assert lib.psm_supported(7, N.FORM_SQRT, 5, 2) == 0
+    # tangent (forward-mode) slices: square-root form only
+    for nx, ny in ((1, 1), (2, 1), (2, 2), (3, 2), (4, 2), (5, 2)):
+        assert lib.psm_supported(N.PSM_F64_TANGENT, N.FORM_SQRT, nx, ny) == 1
+        assert lib.psm_supported(N.PSM_F64_TANGENT, N.FORM_STD, nx, ny) == 0
+    assert lib.psm_supported(N.PSM_F32_TANGENT, N.FORM_SQRT, 5, 2) == 1
+    assert lib.psm_supported(N.PSM_F32_TANGENT, N.FORM_SQRT, 2, 1) == 0
+    assert lib.psm_supported(N.PSM_F64_TANGENT, N.FORM_SQRT, 5, 3) == 0
+
+
+def test_tangent_workspace_is_larger():
+    """Tangent problems store (value, tangent) pairs and FP64 pair log-likelihood terms."""
+    lib = N.load_library()
+    p64 = N.PsmProblem(N.PSM_F64, N.FORM_SQRT, 5, 2, 1, 0, 1025, 1)
+    pd = N.PsmProblem(N.PSM_F64_TANGENT, N.FORM_SQRT, 5, 2, 1, 0, 1025, 1)
+    w64, wd = lib.psm_workspace_bytes(ctypes.byref(p64)), lib.psm_workspace_bytes(ctypes.byref(pd))
+    assert 1.5 * w64 <= wd <= 2.5 * w64
 
 
 def test_workspace_bytes_and_argument_errors():
@@ -56,3 +72,24 @@ def test_workspace_bytes_and_argument_errors():
     assert b"model dims" in lib.psm_last_error() or b"coordinated" in lib.psm_last_error()
     rc = lib.psm_filter_combine(N.PSM_F64, N.FORM_SQRT, 9, 1, None, None, None, None)
     assert rc == -2 and b"unsupported" in lib.psm_last_error()
+
+
+def test_schedule_fields_change_the_plan():
+    """chunk and scan_k of the problem select the launch schedule; the workspace
+    follows it (interleaved buffers are padded to whole 64-chunk groups)."""
+    lib = N.load_library()
+    sizes = {}
+    for chunk in (2, 4, 8, 16, 32):
+        for k in (2, 4, 8):
+            p = N.PsmProblem(N.PSM_F64, N.FORM_SQRT, 5, 2, 1, chunk, 100001, 1)
+            p.scan_k = k
+            assert lib.psm_chunk_size(ctypes.byref(p)) == chunk
+            sizes[chunk, k] = lib.psm_workspace_bytes(ctypes.byref(p))
+            assert sizes[chunk, k] > 100001 * 8 * (71 + 56)
+    assert len(set(sizes.values())) > 1
+    # autotuner argument checks happen before any launch
+    p = N.PsmProblem(N.PSM_F64, N.FORM_SQRT, 5, 2, 1, 0, 1025, 1)
+    m = N.PsmModel(N.MODEL_LINEAR, 5, 5, 0, 1, 1, 1)
+    rc = lib.psm_autotune(ctypes.byref(p), ctypes.byref(m), ctypes.byref(m), None, *([None] * 10), 0, None,
+                          None, None, None)
+    assert rc != 0 and b"null" in lib.psm_last_error()
