@@ -55,7 +55,7 @@ def make_inputs(T, seed=0):
     return ys
 
 
-def build_runner(T, ys, dtype, n_seq=1):
+def build_runner(T, ys, dtype, n_seq=1, method=None):
     import torch
     from parsmooth import FunctionalModel, MVNSqrt
     from parsmooth.batch import IteratedSmoother
@@ -66,7 +66,7 @@ def build_runner(T, ys, dtype, n_seq=1):
     t = lambda a: torch.as_tensor(np.asarray(a, dtype=np.float64)).to(dev, dtype)
     trans = FunctionalModel(trans_f, MVNSqrt(t(np.zeros(5)), t(np.linalg.cholesky(Q))))
     obs = FunctionalModel(obs_f, MVNSqrt(t(np.zeros(2)), t(np.linalg.cholesky(R))))
-    runner = IteratedSmoother([T] * n_seq, trans, obs, extended, dtype=dtype)
+    runner = IteratedSmoother([T] * n_seq, trans, obs, method or extended, dtype=dtype)
     m0 = np.tile(np.array([0.1, 0.2, 1.0, 0.0, 0.0]), (n_seq, 1))
     runner.set_data(t(np.tile(ys, (n_seq, 1))), t(m0), t(np.tile(np.eye(5), (n_seq, 1, 1))))
     M = runner.prob.n_states
@@ -110,10 +110,15 @@ def build_grad_objective(n_seq, T, n_iter, dtype, seed=5):
     rng = np.random.default_rng(seed)
     m0 = np.array([0.1, 0.2, 1.0, 0.0, 0.0]) + np.sqrt(0.1) * rng.standard_normal((n_seq, 5))
     lengths = np.full(n_seq, T)
-    _, obs = bearings_data_batch(lengths, m0[:, :4], 0.01, 0.5, (-1.5, 0.5), (1.0, 1.0), seed=seed + 1)
+    # turn rate simulated as a random walk with the model's own intensity qw = 0.1
+    # (increments of variance qw dt), positions by the exact turn
+    truth, obs = bearings_data_batch(lengths, m0[:, :4], 0.01, 0.5, (-1.5, 0.5), (1.0, 1.0),
+                                     q=np.sqrt(0.1 / 0.01), seed=seed + 1)
     L0 = np.tile(np.eye(5), (n_seq, 1, 1))
     build = bearings_parameterisation(0.01, m0, L0)
-    init = (np.repeat(m0, T + 1, axis=0), np.tile(np.eye(5), (n_seq * (T + 1), 1, 1)))
+    # IPLS started at the simulated trajectories (chol 0.1 I), so that the
+    # fixed point the gradient differentiates is reached in n_iter iterations
+    init = (np.concatenate(truth).astype(np.float64), np.tile(0.1 * np.eye(5), (n_seq * (T + 1), 1, 1)))
     obj = IteratedLoglikelihood(build, N.MODEL_COORDINATED_TURN, N.MODEL_BEARINGS, 5, 2, lengths,
                                 np.concatenate(obs), cubature, n_iter, init, dtype=dtype)
     theta = torch.tensor([np.log(0.01), np.log(0.1), np.log(0.5), -1.5, 0.5, 1.0, 1.0], dtype=torch.float64)
@@ -130,6 +135,107 @@ def time_gradient(obj, theta, steps, warmup):
         ell, grad = obj.value_and_grad(theta)
     torch.cuda.synchronize()
     return (time.perf_counter() - t0) / steps, ell, grad
+
+
+def time_calls(fn, steps, warmup):
+    """The runtime notebook's protocol: warm-up calls, then `steps` timed calls,
+    each synchronised; returns (mean, median) seconds."""
+    import torch
+    for _ in range(warmup):
+        fn()
+    torch.cuda.synchronize()
+    ts = []
+    for _ in range(steps):
+        t0 = time.perf_counter()
+        fn()
+        torch.cuda.synchronize()
+        ts.append(time.perf_counter() - t0)
+    return float(np.mean(ts)), float(np.median(ts))
+
+
+def t_sweep(exps, n_iter, steps, warmup):
+    """Bearings-only IEKS (extended, n_iter iterations) over T = 2^e, FP64 and FP32."""
+    import torch
+    res = []
+    for e in exps:
+        T = 1 << e
+        ys = make_inputs(T)
+        row = {"T": T}
+        for name, dt_ in (("fp64", torch.float64), ("fp32", torch.float32)):
+            r, init = build_runner(T, ys, dt_)
+            mean, med = time_calls(lambda: r.run(n_iter, init, use_graph=True), steps, warmup)
+            row[name] = {"mean_ms": mean * 1e3, "median_ms": med * 1e3, "step_its_per_s": T * n_iter / mean}
+            del r, init
+        res.append(row)
+    torch.cuda.empty_cache()
+    return res
+
+
+def linear_cv_case(steps, warmup):
+    """Linear-Gaussian constant-velocity model (nx=4, ny=2, dt=0.1, q=1, R=0.5^2 I),
+    T=1024, FP64: one parallel filter + smoother pass (latency)."""
+    import torch
+    from parsmooth import FunctionalModel, MVNSqrt
+    from parsmooth.batch import IteratedSmoother
+    from parsmooth.data import cv_model, lgssm_data
+    from parsmooth.linearization import extended
+    from parsmooth.models import LinearFunction
+    T = 1024
+    F, Q, H, R = cv_model(0.1, 1.0, 0.5)
+    rng = np.random.RandomState(0)
+    _, ys = lgssm_data(rng.randn(4), F, H, R, Q, np.zeros(4), np.zeros(2), T, random_state=rng)
+    dev = torch.device("cuda", 0)
+    t = lambda a: torch.as_tensor(np.asarray(a, dtype=np.float64), device=dev)
+    trans = FunctionalModel(LinearFunction(F), MVNSqrt(t(np.zeros(4)), t(np.linalg.cholesky(Q))))
+    obs = FunctionalModel(LinearFunction(H), MVNSqrt(t(np.zeros(2)), t(np.linalg.cholesky(R))))
+    r = IteratedSmoother([T], trans, obs, extended, dtype=torch.float64)
+    r.set_data(t(ys), t(np.zeros((1, 4))), t(np.eye(4)[None]))
+    nom = r.prob.default_nominal()
+    out = r.prob.new_traj()
+    mean, med = time_calls(lambda: r.pass_once(nom, out, use_graph=True), steps, warmup)
+    return {"workload": "linear CV model nx=4 ny=2, T=1024, FP64, one filter+smoother pass (hipGraph)",
+            "mean_us": mean * 1e6, "median_us": med * 1e6}
+
+
+def fp32_cubature_errors(exps, n_iter):
+    """FP32 square-root cubature IPLS vs the FP64 run of the same kernels (which
+    match the FP64 oracle to 1e-9 on the tested sizes) on bearings-only data,
+    per T: one pass at a common nominal and after n_iter iterations; max over
+    states of the mean error and of the L L^T error, relative to the largest
+    magnitude.  FP64 fallback recomputations per pass are reported too."""
+    import torch
+    from parsmooth.linearization import cubature
+    res = []
+    for e in exps:
+        T = 1 << e
+        ys = make_inputs(T, seed=1)
+        r64, init64 = build_runner(T, ys, torch.float64, method=cubature)
+        r32, _ = build_runner(T, ys, torch.float32, method=cubature)
+        # common nominal: the FP64 iterate after n_iter iterations
+        m64, L64 = r64.run(n_iter, init64, use_graph=False)
+        nom = (m64.clone(), L64.clone())
+        a64 = r64.prob.new_traj()
+        a32 = r32.prob.new_traj()
+        r64.pass_once(nom, a64)
+        r32.prob.fallback_counter.zero_()
+        r32.pass_once((nom[0].float(), nom[1].float()), a32)
+        fb = r32.fallback_count()
+        it32 = r32.run(n_iter, (init64[0].float(), init64[1].float()), use_graph=False)
+        torch.cuda.synchronize()
+
+        def rel(a, b):
+            a, b = a.double(), b.double()
+            return float((a - b).abs().max() / max(1.0, float(b.abs().max())))
+
+        def gram(L):
+            L = L.double()
+            return L @ L.transpose(-1, -2)
+        res.append({"T": T, "pass_mean": rel(a32[0], a64[0]), "pass_cov": rel(gram(a32[1]), gram(a64[1])),
+                    "iterated_mean": rel(it32[0], m64), "iterated_cov": rel(gram(it32[1]), gram(L64)),
+                    "fallbacks_per_pass": fb})
+        del r64, r32, init64, nom, a64, a32
+        torch.cuda.empty_cache()
+    return res
 
 
 def time_runner(runner, init, n_iter, steps, warmup, use_graph, kernel_timing=False):
@@ -200,6 +306,10 @@ def main():
     ap.add_argument("--grad-seqs", type=int, default=256)
     ap.add_argument("--grad-T", type=int, default=16384)
     ap.add_argument("--grad-iter", type=int, default=10)
+    ap.add_argument("--sweep", type=str, default="10,12,14,16",
+                    help="comma-separated log2 T of the IEKS T-sweep ('' to skip)")
+    ap.add_argument("--fp32-errors", type=str, default="14,16,18",
+                    help="comma-separated log2 T of the FP32 cubature error report ('' to skip)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run one 256 MiB device copy first (byte-counter calibration)")
     args = ap.parse_args()
@@ -268,24 +378,43 @@ def main():
         for name, dt_ in (("fp32", torch.float32), ("fp64", torch.float64)):
             log(f"gradient leg {name}: {args.grad_seqs} x T={args.grad_T}")
             obj, theta = build_grad_objective(args.grad_seqs, args.grad_T, args.grad_iter, dt_)
+            obj.prob.fallback_counter.zero_()
+            obj.tprob.fallback_counter.zero_()
             tg, ell_g, g_g = time_gradient(obj, theta, 1, 1 if name == "fp32" else 0)
-            legs[name] = (tg, ell_g.double().cpu().numpy(), g_g.double().cpu().numpy())
+            fbk = int(obj.prob.fallback_counter.item()) + int(obj.tprob.fallback_counter.item())
+            res = obj.fixed_point_residual(theta) if name == "fp64" else None
+            legs[name] = (tg, ell_g.double().cpu().numpy(), g_g.double().cpu().numpy(), res, fbk)
             del obj
             torch.cuda.empty_cache()
-        g32, g64 = legs["fp32"][2].sum(0), legs["fp64"][2].sum(0)
+        conv = legs["fp64"][3] < 1e-6
+        g32, g64 = legs["fp32"][2][conv].sum(0), legs["fp64"][2][conv].sum(0)
+        e32, e64 = legs["fp32"][1][conv].sum(), legs["fp64"][1][conv].sum()
         nst = args.grad_seqs * args.grad_T
         grad = {"workload": f"{args.grad_seqs} bearings-only sequences x T={args.grad_T}, cubature IPLS, "
-                            f"n_iter={args.grad_iter}, gradient of the summed pseudo-log-likelihood in 7 "
-                            "parameters (implicit fixed-point rule, forward-mode tangent passes)",
+                            f"n_iter={args.grad_iter} from the simulated trajectories, gradient of the summed "
+                            "pseudo-log-likelihood in 7 parameters (implicit fixed-point rule, "
+                            "forward-mode tangent passes)",
                 "fp32_grad_evals_per_s": args.grad_seqs / legs["fp32"][0],
                 "fp64_grad_evals_per_s": args.grad_seqs / legs["fp64"][0],
                 "fp32_ms": legs["fp32"][0] * 1e3, "fp64_ms": legs["fp64"][0] * 1e3,
                 "fp32_over_fp64": legs["fp64"][0] / legs["fp32"][0],
                 "fp32_smoothed_step_its_per_s": nst * args.grad_iter / legs["fp32"][0],
+                "fp64_sequences_at_fixed_point": int(conv.sum()),
+                "fixed_point_criterion": "max |f(x*) - x*| over the state means < 1e-6 after n_iter iterations",
                 "fp32_grad_rel_err_vs_fp64": float(np.abs(g32 - g64).max() / np.abs(g64).max()),
-                "fp32_ell_rel_err_vs_fp64": float(np.abs(legs["fp32"][1].sum() - legs["fp64"][1].sum())
-                                                  / abs(legs["fp64"][1].sum())),
+                "fp32_ell_rel_err_vs_fp64": float(abs(e32 - e64) / abs(e64)),
+                "fp32_fp64_fallbacks": legs["fp32"][4],
+                "errors_over": "the sequences at a fixed point (summed gradient, summed ell)",
                 "grad_fp64": g64.tolist()}
+
+    sweep = lin = errs = None
+    if args.sweep:
+        log(f"T sweep {args.sweep}")
+        sweep = t_sweep([int(x) for x in args.sweep.split(",")], n_iter, args.steps, args.warmup)
+        lin = linear_cv_case(max(args.steps, 20), args.warmup)
+    if args.fp32_errors:
+        log(f"FP32 cubature errors {args.fp32_errors}")
+        errs = fp32_cubature_errors([int(x) for x in args.fp32_errors.split(",")], 5)
 
     value = T * n_iter / dt64
     # dominant kernel by total time
@@ -319,6 +448,9 @@ def main():
         "kernel_ms": {k: v[0] / v[1] for k, v in rep.items()},
         "batch_fp32_cubature": batch,
         "gradient": grad,
+        "t_sweep_ieks": sweep,
+        "linear_cv_T1024": lin,
+        "fp32_cubature_error_per_T": errs,
         "roofline": roof,
         "cpu_baseline": cpu,
     }

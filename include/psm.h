@@ -81,6 +81,9 @@ typedef struct psm_sigma {
   const void* wc;  /* (npts,) */
 } psm_sigma_t;
 
+#define PSM_SCAN_K_MASK 0xff
+#define PSM_SCAN_TEAM 0x100
+
 /* A ragged batch of sequences sharing dims, models and linearization. */
 typedef struct psm_problem {
   int32_t dtype;
@@ -98,7 +101,10 @@ typedef struct psm_problem {
    * disables the check.  fallback_count (device, may be NULL) is incremented
    * once per recomputation; the caller zeroes it. */
   float fallback_ratio;
-  int32_t scan_k;        /* scan items folded per lane (0: PSM_SCAN_K or 4) */
+  /* bits 0-7: scan items folded per lane (0: PSM_SCAN_K or 4);
+   * PSM_SCAN_TEAM: levels above the first run team-cooperative kernels
+   * (square-root form, float / double) */
+  int32_t scan_k;
   unsigned long long* fallback_count;
 } psm_problem_t;
 
@@ -196,7 +202,8 @@ int psm_mean_sq_diff(int32_t dtype, int64_t n, const void* a, const void* b, dou
 /* Schedule autotuner (no reference counterpart: the reference's schedule is
  * XLA's).  Times psm_filter_smoother (eager, on `stream`) for every candidate
  * schedule -- chunk in {2, 4, 8, 16, 32} states per lane, scan fan-in
- * scan_k in {2, 4, 8} -- whose workspace fits workspace_bytes, and returns the
+ * scan_k in {2, 4, 8}, with and without PSM_SCAN_TEAM (square-root form in
+ * float / double) -- whose workspace fits workspace_bytes, and returns the
  * fastest in *best_chunk / *best_scan_k (and its time per pass in *best_ms,
  * may be NULL).  prob->chunk / prob->scan_k are ignored.  The outputs
  * (out_m, out_L, filt_m, filt_L) are overwritten; FP32 fallback counts

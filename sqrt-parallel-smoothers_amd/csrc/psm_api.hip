@@ -454,9 +454,12 @@ int psm_autotune(const psm_problem_t* p, const psm_model_t* trans, const psm_mod
   TuneResult best{0, 0, 0.0f};
   static const int32_t chunks[] = {2, 4, 8, 16, 32};
   static const int32_t ks[] = {2, 4, 8};
+  const bool team_ok = p->form == PSM_FORM_SQRT && (p->dtype == PSM_F32 || p->dtype == PSM_F64);
   constexpr int kReps = 2;
   for (int32_t c : chunks) {
-    for (int32_t k : ks) {
+    for (int32_t kk = 0; kk < 6; ++kk) {
+      const int32_t k = ks[kk % 3] | (kk >= 3 ? PSM_SCAN_TEAM : 0);
+      if (kk >= 3 && !team_ok) continue;
       psm_problem_t q = *p;
       q.chunk = c;
       q.scan_k = k;

@@ -29,6 +29,8 @@ struct DualT {
   PSM_DEV constexpr DualT(float x) : v((B)x), d(B(0)) {}
   PSM_DEV constexpr DualT(int x) : v((B)x), d(B(0)) {}
   PSM_DEV constexpr DualT(B x, B y) : v(x), d(y) {}
+  template <typename B2>
+  PSM_DEV explicit constexpr DualT(const DualT<B2>& o) : v((B)o.v), d((B)o.d) {}
   PSM_DEV explicit operator int() const { return (int)v; }
   PSM_DEV DualT& operator+=(const DualT& o) { v += o.v; d += o.d; return *this; }
   PSM_DEV DualT& operator-=(const DualT& o) { v -= o.v; d -= o.d; return *this; }

@@ -41,6 +41,7 @@ struct ScanLevel {
   int K;          // items per lane
   int64_t nwg;    // workgroups
   int64_t stride; // padded item count (component stride of the interleaved layout)
+  int wg;         // lanes (or teams) per scan workgroup at this level
   size_t off_in;  // byte offsets in the workspace
   size_t off_out;
 };
@@ -50,6 +51,7 @@ struct Plan {
   int64_t Mp = 0;  // padded state count (component stride of the per-state buffers)
   int chunk = 0;
   int wg = 0;      // scan workgroup size
+  bool team = false;  // team-cooperative kernels above level 0
   int64_t nchunks = 0;
   int nlevels = 0;
   ScanLevel lv[16];
@@ -71,8 +73,11 @@ inline int default_chunk(int64_t M) {
   return 4;
 }
 
+// wg: lanes per workgroup of level 0 (one lane per combination); wgu: lanes or
+// teams per workgroup of the levels above.
 inline Plan make_plan(const psm_problem_t* p, size_t rsize, int nlin, int nqf, int nqs,
-                      size_t accsize = sizeof(double), int wg = kScanWG) {
+                      size_t accsize = sizeof(double), int wg = kScanWG, int wgu = 0) {
+  if (wgu <= 0) wgu = wg;
   Plan pl;
   pl.M = p->n_states;
   pl.chunk = p->chunk > 0 ? p->chunk : env_int("PSM_CHUNK", default_chunk(pl.M));
@@ -87,21 +92,24 @@ inline Plan make_plan(const psm_problem_t* p, size_t rsize, int nlin, int nqf, i
   off = align_up(off + (size_t)pl.M * accsize);
   pl.off_selem = off;
   off = align_up(off + (size_t)nqs * pl.Mp * rsize);
-  const int kmax = p->scan_k > 0 ? p->scan_k : env_int("PSM_SCAN_K", 4);
+  const int kreq = p->scan_k & PSM_SCAN_K_MASK;
+  const int kmax = kreq > 0 ? kreq : env_int("PSM_SCAN_K", 4);
   int64_t n = pl.nchunks;
   int l = 0;
   while (true) {
     ScanLevel& s = pl.lv[l];
     s.n = n;
-    if (n <= (int64_t)wg * 16) {
-      s.K = (int)((n + wg - 1) / wg);
+    s.wg = l == 0 ? wg : wgu;
+    const int wg_l = s.wg;
+    if (n <= (int64_t)wg_l * 16) {
+      s.K = (int)((n + wg_l - 1) / wg_l);
       if (s.K < 1) s.K = 1;
       s.nwg = 1;
     } else {
       s.K = kmax;
-      s.nwg = (n + (int64_t)wg * kmax - 1) / ((int64_t)wg * kmax);
+      s.nwg = (n + (int64_t)wg_l * kmax - 1) / ((int64_t)wg_l * kmax);
     }
-    const int64_t blk = (int64_t)wg * s.K;
+    const int64_t blk = (int64_t)wg_l * s.K;
     s.stride = (n + blk - 1) / blk * blk;
     s.off_in = off;
     off = align_up(off + (size_t)nq * s.stride * rsize);
@@ -172,16 +180,24 @@ struct Impl {
   static constexpr int NLIN = Step<R, NX, NY>::NQ;
 
   using Acc = typename AccOf<R>::type;
-  // Square-root form in float / double: team-cooperative scan kernels (WG =
-  // teams per workgroup); otherwise one lane per combination (WG = lanes).
-  static constexpr bool TEAM = kTeamScan && SQRT &&
-                               (std::is_same<R, float>::value || std::is_same<R, double>::value);
-  static constexpr int WG =
-      TEAM ? kTeamVL
-           : (scan_wg<FA, R>() < scan_wg<SA, R>() ? scan_wg<FA, R>() : scan_wg<SA, R>());
+  // Level 0 of a scan (the chunk aggregates: many items) runs one lane per
+  // combination with WG lanes per workgroup.  On request (PSM_SCAN_TEAM in
+  // scan_k, or the PSM_SCAN_TEAM environment variable), the square-root form
+  // in float / double runs the levels above (few items, a latency chain) with
+  // the team-cooperative kernels, kTeamVL teams per workgroup.
+  static constexpr bool TEAM = SQRT && (std::is_same<R, float>::value ||
+                                        std::is_same<R, double>::value);
+  static constexpr int WG = scan_wg<FA, R>() < scan_wg<SA, R>() ? scan_wg<FA, R>() : scan_wg<SA, R>();
+
+  static bool team_requested(const psm_problem_t* p) {
+    return (p->scan_k & PSM_SCAN_TEAM) != 0 || env_int("PSM_SCAN_TEAM", 0) != 0;
+  }
 
   static Plan plan(const psm_problem_t* p) {
-    return make_plan(p, sizeof(R), NLIN, FA::NQ, SA::NQ, sizeof(Acc), WG);
+    const bool team = TEAM && team_requested(p);
+    Plan pl = make_plan(p, sizeof(R), NLIN, FA::NQ, SA::NQ, sizeof(Acc), WG, team ? kTeamVL : WG);
+    pl.team = team;
+    return pl;
   }
 
   static size_t workspace_bytes(const psm_problem_t* p) { return plan(p).total; }
@@ -265,7 +281,7 @@ struct Impl {
   // offset of the complete exclusive prefixes.
   static LevelMap level_map(const Plan& pl, int l, int rev) {
     const ScanLevel& s = pl.lv[l];
-    return LevelMap{s.n, s.stride, s.K, WG, rev};
+    return LevelMap{s.n, s.stride, s.K, s.wg, rev};
   }
 
   // Exclusive scan of the level-0 items (chunk aggregates, already in scan
@@ -284,11 +300,16 @@ struct Impl {
       const LevelMap tm = top ? LevelMap{0, 0, 1, WG, 0} : level_map(pl, l + 1, 0);
       TimedScope ts_(l == 0 ? (filt ? "k_fscan_up_l0" : "k_sscan_up_l0")
                             : (filt ? "k_fscan_up" : "k_sscan_up"), st);
-      if constexpr (TEAM)
-        hipLaunchKernelGGL((k_scan_up_team<Agg, R>), dim3((unsigned)s.nwg),
-                           dim3(TeamCfg<Agg, R>::WGT), 0, st, (const R*)in, level_map(pl, l, 0), out,
-                           tot, tm, fb);
-      else
+      bool launched = false;
+      if constexpr (TEAM) {
+        if (l > 0 && pl.team) {
+          hipLaunchKernelGGL((k_scan_up_team<Agg, R>), dim3((unsigned)s.nwg),
+                             dim3(TeamCfg<Agg, R>::WGT), 0, st, (const R*)in, level_map(pl, l, 0),
+                             out, tot, tm, fb);
+          launched = true;
+        }
+      }
+      if (!launched)
         hipLaunchKernelGGL((k_scan_up<Agg, R, SQRT, WG>), dim3((unsigned)s.nwg), dim3(WG), 0,
                            st, (const R*)in, level_map(pl, l, 0), out, tot, tm, fb);
       int rc = hip_check("k_scan_up");
@@ -300,12 +321,17 @@ struct Impl {
       const size_t upper_off =
           (l + 1 == pl.nlevels - 1) ? pl.lv[l + 1].off_out : pl.lv[l + 1].off_in;
       TimedScope ts_(filt ? "k_fscan_down" : "k_sscan_down", st);
-      if constexpr (TEAM)
-        hipLaunchKernelGGL((k_scan_down_team<Agg, R>), dim3((unsigned)s.nwg),
-                           dim3(TeamCfg<Agg, R>::WGT), 0, st, (const R*)(ws + s.off_out),
-                           level_map(pl, l, 0), (const R*)(ws + upper_off), level_map(pl, l + 1, 0),
-                           (R*)(ws + s.off_in), fb);
-      else
+      bool launched = false;
+      if constexpr (TEAM) {
+        if (l > 0 && pl.team) {
+          hipLaunchKernelGGL((k_scan_down_team<Agg, R>), dim3((unsigned)s.nwg),
+                             dim3(TeamCfg<Agg, R>::WGT), 0, st, (const R*)(ws + s.off_out),
+                             level_map(pl, l, 0), (const R*)(ws + upper_off),
+                             level_map(pl, l + 1, 0), (R*)(ws + s.off_in), fb);
+          launched = true;
+        }
+      }
+      if (!launched)
         hipLaunchKernelGGL((k_scan_down<Agg, R, SQRT, WG>), dim3((unsigned)s.nwg), dim3(WG), 0,
                            st, (const R*)(ws + s.off_out), level_map(pl, l, 0),
                            (const R*)(ws + upper_off), level_map(pl, l + 1, 0),
@@ -343,12 +369,12 @@ struct Impl {
     if (pl.M <= 0) return PSM_OK;
     char* ws = (char*)wsv;
     int rc;
+    const Fallback fb{p->fallback_ratio, p->fallback_count};
     if (!lin_done) {
       rc = linearize_states(pl, tr, ob, sg, nm, nL, ws, st);
       if (rc) return rc;
     }
     const R* lin = (const R*)(ws + pl.off_lin);
-    const Fallback fb{p->fallback_ratio, p->fallback_count};
     {
     TimedScope ts_("k_filter_reduce", st);
     hipLaunchKernelGGL((k_filter_reduce<R, NX, NY, SQRT>), dim3(grid_for(pl.nchunks, kLaneWG)),
@@ -388,11 +414,11 @@ struct Impl {
     if (pl.M <= 0) return PSM_OK;
     char* ws = (char*)wsv;
     int rc;
+    const Fallback fb{p->fallback_ratio, p->fallback_count};
     if (!lin_done) {
       rc = linearize_states(pl, tr, nullptr, sg, nm, nL, ws, st);
       if (rc) return rc;
     }
-    const Fallback fb{p->fallback_ratio, p->fallback_count};
     {
     TimedScope ts_("k_smoother_reduce", st);
     hipLaunchKernelGGL((k_smoother_reduce<R, NX, NY, SQRT>), dim3(grid_for(pl.nchunks, kLaneWG)),

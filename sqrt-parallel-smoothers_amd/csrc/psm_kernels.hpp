@@ -149,6 +149,49 @@ PSM_DEV void linearize_point(const ModelView<R>& md, const SigmaView<R>& sp, int
 
 // Thread t fills position t of the state-interleaved buffer (coalesced stores);
 // its state is the inverse of StateMap.
+// Sigma-point linearization of the FP32 square-root path in FP64 arithmetic
+// (FP32 in and out): the statistical linear regression subtracts F P F^T from
+// the sigma-point covariance by rank-one downdates, and when the noise is
+// small against the sigma-point spread (chol Q ~ 6e-5 against a spread ~ 0.1
+// on the bearings-only model) those downdates cancel in FP32 -- the factor
+// loses all its accuracy or the downdate fails outright.
+template <typename R> struct WiderOf { using type = R; };
+template <> struct WiderOf<float> { using type = double; };
+template <> struct WiderOf<DualT<float>> { using type = DualT<double>; };
+
+template <typename A, int N, typename B>
+PSM_DEV void widen(const B (&a)[N], A (&b)[N]) {
+#pragma unroll
+  for (int i = 0; i < N; ++i) b[i] = A(a[i]);
+}
+template <typename A, int M, int N, typename B>
+PSM_DEV void widen(const B (&a)[M][N], A (&b)[M][N]) {
+#pragma unroll
+  for (int i = 0; i < M; ++i) widen(a[i], b[i]);
+}
+
+template <typename R, int NIN, int NOUT, bool SQRT>
+PSM_DEV void linearize_point_w(const ModelView<R>& md, const SigmaView<R>& sp, int use_sigma,
+                               const R (&m)[NIN], const R (&Lx)[NIN][NIN], R (&F)[NOUT][NIN],
+                               R (&C)[NOUT][NOUT], R (&b)[NOUT]) {
+  using W = typename WiderOf<R>::type;
+  if constexpr (SQRT && !std::is_same<W, R>::value) {
+    if (use_sigma) {
+      const ModelView<W, R> mw{md.kind, md.params, md.q_mean, md.q_chol};
+      const SigmaView<W, R> sw{sp.npts, sp.xi, sp.wm, sp.wc};
+      W mW[NIN], LW[NIN][NIN], FW[NOUT][NIN], CW[NOUT][NOUT], bW[NOUT];
+      widen(m, mW);
+      widen(Lx, LW);
+      linearize_sigma<W, NIN, NOUT, true>(mw, sw, mW, LW, FW, CW, bW);
+      widen(FW, F);
+      widen(CW, C);
+      widen(bW, b);
+      return;
+    }
+  }
+  linearize_point<R, NIN, NOUT, SQRT>(md, sp, use_sigma, m, Lx, F, C, b);
+}
+
 template <typename R, int NX, int NY, bool SQRT>
 __global__ __launch_bounds__(kLaneWG) void k_linearize_states(
     int64_t M, int64_t Mp, int C, ModelView<R> tr, ModelView<R> ob, int do_obs, SigmaView<R> sp,
@@ -165,7 +208,7 @@ __global__ __launch_bounds__(kLaneWG) void k_linearize_states(
   ld_aos(nom_L, i, L);
   {
     R F[NX][NX], Cq[NX][NX], b[NX];
-    linearize_point<R, NX, NX, SQRT>(tr, sp, use_sigma, m, L, F, Cq, b);
+    linearize_point_w<R, NX, NX, SQRT>(tr, sp, use_sigma, m, L, F, Cq, b);
     int q = 0;
     st(lin, Mp, t, q, F);
     st(lin, Mp, t, q, Cq);
@@ -173,7 +216,7 @@ __global__ __launch_bounds__(kLaneWG) void k_linearize_states(
   }
   if (do_obs) {
     R H[NY][NX], Cr[NY][NY], c[NY];
-    linearize_point<R, NX, NY, SQRT>(ob, sp, use_sigma, m, L, H, Cr, c);
+    linearize_point_w<R, NX, NY, SQRT>(ob, sp, use_sigma, m, L, H, Cr, c);
     int q = Step<R, NX, NY>::NTRANS;
     st(lin, Mp, t, q, H);
     st(lin, Mp, t, q, Cr);
@@ -309,10 +352,10 @@ __global__ __launch_bounds__(WG) void k_scan_down(const R* __restrict__ out_excl
 template <typename Agg, typename R>
 struct TeamCfg {
   using V = typename ViewOf<Agg>::type;
-  static constexpr int TL = TeamOf<V>::TL;
+  static constexpr int TL = kTeamLanes;
   static constexpr int VL = kTeamVL;
   static constexpr int WGT = VL * TL;
-  static constexpr int SCR = team_scratch_bytes<V, R>();
+  static constexpr int SCR = team_scratch_bytes<V>();
 };
 
 template <typename Agg, typename R>
@@ -554,7 +597,7 @@ __global__ __launch_bounds__(kLaneWG) void k_linearize_points(
   R m[NIN], L[NIN][NIN], Fo[NOUT][NIN], Co[NOUT][NOUT], bo[NOUT];
   ld_aos(xm, i, m);
   ld_aos(xL, i, L);
-  linearize_point<R, NIN, NOUT, SQRT>(md, sp, use_sigma, m, L, Fo, Co, bo);
+  linearize_point_w<R, NIN, NOUT, SQRT>(md, sp, use_sigma, m, L, Fo, Co, bo);
   st_aos(F, i, Fo);
   st_aos(C, i, Co);
   st_aos(b, i, bo);

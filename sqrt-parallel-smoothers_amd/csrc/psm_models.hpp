@@ -13,12 +13,17 @@
 namespace psm {
 
 // Device view of a psm_model_t (see include/psm.h).
-template <typename R>
+// S is the storage type of the arrays, R the arithmetic type the values are
+// read as (a wider R recomputes an FP32 linearization in FP64).
+template <typename R, typename S = R>
 struct ModelView {
   int kind;
-  const R* params;   // model parameters (device)
-  const R* q_mean;   // additive noise mean (device, nout)
-  const R* q_chol;   // noise chol (sqrt form) or covariance (covariance form), nout x nout
+  const S* params;   // model parameters (device)
+  const S* q_mean;   // additive noise mean (device, nout)
+  const S* q_chol;   // noise chol (sqrt form) or covariance (covariance form), nout x nout
+  PSM_DEV R par(int i) const { return R(params[i]); }
+  PSM_DEV R qm(int i) const { return R(q_mean[i]); }
+  PSM_DEV R qc(int i) const { return R(q_chol[i]); }
 };
 
 PSM_DEV double atan2_(double y, double x) { return atan2(y, x); }
@@ -46,8 +51,8 @@ template <typename B> PSM_DEV DualT<B> atan2_(DualT<B> y, DualT<B> x) {
 }
 
 // f(x) and, when JAC, its Jacobian J (NOUT x NIN).
-template <typename R, int NIN, int NOUT, bool JAC>
-PSM_DEV void model_eval(const ModelView<R>& md, const R (&x)[NIN], R (&y)[NOUT], R (&J)[NOUT][NIN]) {
+template <typename R, int NIN, int NOUT, bool JAC, typename S>
+PSM_DEV void model_eval(const ModelView<R, S>& md, const R (&x)[NIN], R (&y)[NOUT], R (&J)[NOUT][NIN]) {
   switch (md.kind) {
     case PSM_MODEL_LINEAR: {
       // x -> A x, A row-major NOUT x NIN
@@ -56,7 +61,7 @@ PSM_DEV void model_eval(const ModelView<R>& md, const R (&x)[NIN], R (&y)[NOUT],
         R s = R(0);
 #pragma unroll
         for (int j = 0; j < NIN; ++j) {
-          const R a = md.params[i * NIN + j];
+          const R a = md.par(i * NIN + j);
           s = fma(a, x[j], s);
           if (JAC) J[i][j] = a;
         }
@@ -68,7 +73,7 @@ PSM_DEV void model_eval(const ModelView<R>& md, const R (&x)[NIN], R (&y)[NOUT],
       if constexpr (NIN == 5 && NOUT == 5) {
         // tests/bearings/bearings_utils.py:7-44: branch at |w| < 1e-6 where
         // sin(wt)/w -> dt and (cos(wt)-1)/w -> 0 with zero w-derivative.
-        const R dt = md.params[0];
+        const R dt = md.par(0);
         const R w = x[4];
         const R c = cos_(w * dt);
         const R s = sin_(w * dt);
@@ -102,8 +107,8 @@ PSM_DEV void model_eval(const ModelView<R>& md, const R (&x)[NIN], R (&y)[NOUT],
         // tests/bearings/bearings_utils.py:47-69 (no angle wrapping)
 #pragma unroll
         for (int r = 0; r < 2; ++r) {
-          const R dx = x[0] - md.params[2 * r];
-          const R dy = x[1] - md.params[2 * r + 1];
+          const R dx = x[0] - md.par(2 * r);
+          const R dy = x[1] - md.par(2 * r + 1);
           y[r] = atan2_(dy, dx);
           if (JAC) {
             const R q = dx * dx + dy * dy;
@@ -139,19 +144,22 @@ PSM_DEV void model_eval(const ModelView<R>& md, const R (&x)[NIN], R (&y)[NOUT],
   }
 }
 
-template <typename R, int NIN, int NOUT>
-PSM_DEV void model_value(const ModelView<R>& md, const R (&x)[NIN], R (&y)[NOUT]) {
+template <typename R, int NIN, int NOUT, typename S>
+PSM_DEV void model_value(const ModelView<R, S>& md, const R (&x)[NIN], R (&y)[NOUT]) {
   R J[NOUT][NIN];
   model_eval<R, NIN, NOUT, false>(md, x, y, J);
 }
 
 // Sigma-point table (unit points xi_p, weights) in device memory.
-template <typename R>
+template <typename R, typename S = R>
 struct SigmaView {
   int npts;
-  const R* xi;  // npts x n
-  const R* wm;
-  const R* wc;
+  const S* xi;  // npts x n
+  const S* wm;
+  const S* wc;
+  PSM_DEV R pxi(int i) const { return R(xi[i]); }
+  PSM_DEV R pwm(int i) const { return R(wm[i]); }
+  PSM_DEV R pwc(int i) const { return R(wc[i]); }
 };
 
 // ----------------------------------------------------------------------------
@@ -159,8 +167,8 @@ struct SigmaView {
 // F = df/dx(m), chol/cov passed through, b = f(m) - F m + m_q.  The nominal
 // covariance is not used.
 // ----------------------------------------------------------------------------
-template <typename R, int NIN, int NOUT>
-PSM_DEV void linearize_extended(const ModelView<R>& md, const R (&m)[NIN], R (&F)[NOUT][NIN],
+template <typename R, int NIN, int NOUT, typename S>
+PSM_DEV void linearize_extended(const ModelView<R, S>& md, const R (&m)[NIN], R (&F)[NOUT][NIN],
                                 R (&C)[NOUT][NOUT], R (&b)[NOUT]) {
   R y[NOUT];
   model_eval<R, NIN, NOUT, true>(md, m, y, F);
@@ -169,9 +177,9 @@ PSM_DEV void linearize_extended(const ModelView<R>& md, const R (&m)[NIN], R (&F
     R s = y[i];
 #pragma unroll
     for (int j = 0; j < NIN; ++j) s = fma(-F[i][j], m[j], s);
-    b[i] = s + md.q_mean[i];
+    b[i] = s + md.qm(i);
 #pragma unroll
-    for (int j = 0; j < NOUT; ++j) C[i][j] = md.q_chol[i * NOUT + j];
+    for (int j = 0; j < NOUT; ++j) C[i][j] = md.qc(i * NOUT + j);
   }
 }
 
@@ -185,8 +193,8 @@ PSM_DEV void linearize_extended(const ModelView<R>& md, const R (&m)[NIN], R (&F
 // downdated by the columns of F L (rank-one downdates, multiplier -1).
 // SQRT=false returns Phi - F P F^T + Q symmetrised (:80-85).
 // ----------------------------------------------------------------------------
-template <typename R, int NIN, int NOUT, bool SQRT>
-PSM_DEV void linearize_sigma(const ModelView<R>& md, const SigmaView<R>& sp, const R (&m)[NIN],
+template <typename R, int NIN, int NOUT, bool SQRT, typename S>
+PSM_DEV void linearize_sigma(const ModelView<R, S>& md, const SigmaView<R, S>& sp, const R (&m)[NIN],
                              const R (&L)[NIN][NIN], R (&F)[NOUT][NIN], R (&C)[NOUT][NOUT],
                              R (&b)[NOUT]) {
   R mf[NOUT];
@@ -197,12 +205,12 @@ PSM_DEV void linearize_sigma(const ModelView<R>& md, const SigmaView<R>& sp, con
     for (int i = 0; i < NIN; ++i) {
       R s = m[i];
 #pragma unroll
-      for (int j = 0; j < NIN; ++j) s = fma(L[i][j], sp.xi[p * NIN + j], s);
+      for (int j = 0; j < NIN; ++j) s = fma(L[i][j], sp.pxi(p * NIN + j), s);
       chi[i] = s;
     }
     R f[NOUT];
     model_value<R, NIN, NOUT>(md, chi, f);
-    const R w = sp.wm[p];
+    const R w = sp.pwm(p);
 #pragma unroll
     for (int i = 0; i < NOUT; ++i) mf[i] = fma(w, f[i], mf[i]);
   }
@@ -214,7 +222,7 @@ PSM_DEV void linearize_sigma(const ModelView<R>& md, const SigmaView<R>& sp, con
 #pragma unroll
     for (int i = 0; i < NOUT; ++i)
 #pragma unroll
-      for (int j = 0; j < NOUT; ++j) Acc[i][j] = md.q_chol[i * NOUT + j];
+      for (int j = 0; j < NOUT; ++j) Acc[i][j] = md.qc(i * NOUT + j);
     {
       R sq[NOUT][NOUT];
 #pragma unroll
@@ -236,13 +244,13 @@ PSM_DEV void linearize_sigma(const ModelView<R>& md, const SigmaView<R>& sp, con
     for (int i = 0; i < NIN; ++i) {
       R s = m[i];
 #pragma unroll
-      for (int j = 0; j < NIN; ++j) s = fma(L[i][j], sp.xi[p * NIN + j], s);
+      for (int j = 0; j < NIN; ++j) s = fma(L[i][j], sp.pxi(p * NIN + j), s);
       chi[i] = s;
       dx[i] = s - m[i];
     }
     R f[NOUT];
     model_value<R, NIN, NOUT>(md, chi, f);
-    const R w = sp.wc[p];
+    const R w = sp.pwc(p);
     R d[NOUT];
 #pragma unroll
     for (int i = 0; i < NOUT; ++i) d[i] = f[i] - mf[i];
@@ -277,7 +285,7 @@ PSM_DEV void linearize_sigma(const ModelView<R>& md, const SigmaView<R>& sp, con
     R s = mf[i];
 #pragma unroll
     for (int j = 0; j < NIN; ++j) s = fma(-F[i][j], m[j], s);
-    b[i] = s + md.q_mean[i];
+    b[i] = s + md.qm(i);
   }
   R FL[NOUT][NIN];
   mm<R, NOUT, NIN, NIN>(F, L, FL);
@@ -299,7 +307,7 @@ PSM_DEV void linearize_sigma(const ModelView<R>& md, const SigmaView<R>& sp, con
     for (int i = 0; i < NOUT; ++i)
 #pragma unroll
       for (int j = 0; j < NOUT; ++j) {
-        R s = Phi[i][j] + md.q_chol[i * NOUT + j];
+        R s = Phi[i][j] + md.qc(i * NOUT + j);
 #pragma unroll
         for (int k = 0; k < NIN; ++k) s = fma(-FL[i][k], FL[j][k], s);
         C[i][j] = s;

@@ -12,6 +12,7 @@ namespace psm {
 
 template <typename R, int NX>
 struct FView {
+  using real_type = R;
   R* base;
   int64_t stride;
   int64_t idx;
@@ -29,6 +30,7 @@ struct FView {
 
 template <typename R, int NX>
 struct SView {
+  using real_type = R;
   R* base;
   int64_t stride;
   int64_t idx;

@@ -69,7 +69,7 @@ class Problem:
             raise NotImplementedError(f"no compiled kernels for nx={nx}, ny={ny}, dtype code {code}")
         # FP64 re-triangularisation fallback of the FP32 square-root path
         self.fallback_counter = torch.zeros(1, dtype=torch.int64, device=dev)
-        ratio = FP32_FALLBACK_RATIO if (code == N.PSM_F32 and sqrt) else 0.0
+        ratio = FP32_FALLBACK_RATIO if (code in (N.PSM_F32, N.PSM_F32_TANGENT) and sqrt) else 0.0
         self.struct = N.PsmProblem(code, N.FORM_SQRT if sqrt else N.FORM_STD,
                                    self.nx, self.ny, self.n_seq, int(chunk), self.n_states,
                                    self.state_offsets.data_ptr(), float(ratio), 0,
@@ -79,9 +79,10 @@ class Problem:
             N.check(-1, "psm_workspace_bytes")
         self._ws = None
 
-    # candidate schedules of the library's autotuner (chunk, scan fan-in)
+    # candidate schedules of the library's autotuner: chunk, and scan fan-in
+    # with or without the team-cooperative upper scan levels (N.SCAN_TEAM)
     TUNE_CHUNKS = (2, 4, 8, 16, 32)
-    TUNE_SCAN_K = (2, 4, 8)
+    TUNE_SCAN_K = (2, 4, 8, 2 | 0x100, 4 | 0x100, 8 | 0x100)
 
     def set_schedule(self, chunk: int, scan_k: int):
         """Fix the launch schedule (0 = library default) and resize the workspace."""

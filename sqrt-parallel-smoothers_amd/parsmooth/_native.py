@@ -15,6 +15,7 @@ import torch
 
 PSM_F32, PSM_F64 = 0, 1
 PSM_F32_TANGENT, PSM_F64_TANGENT = 2, 3
+SCAN_K_MASK, SCAN_TEAM = 0xFF, 0x100   # psm_problem_t.scan_k bits
 FORM_SQRT, FORM_STD = 0, 1
 
 MODEL_LINEAR = 0

@@ -148,6 +148,21 @@ class IteratedLoglikelihood:
         self._filter_ell(self.prob, models, sigma, self.ys, m0, L0, cur, filt, ell)
         return ell, cur[0], cur[1]
 
+    def fixed_point_residual(self, theta: torch.Tensor) -> np.ndarray:
+        """Per sequence, max |f(x*) - x*| over the means of its states: how far
+        the n_iter-th iterate is from a fixed point.  The implicit gradient is the
+        derivative of the converged log-likelihood only where this is small (an
+        iteration caught in a cycle has no fixed point to differentiate)."""
+        theta = theta.detach().to(torch.float64).cpu()
+        _, xm, xL = self.value(theta)
+        models = _ModelSet(self.kinds, (self.nx, self.ny), self.build(theta), self.dev, self.dtype)
+        sigma, _keep = self._sigma(False)
+        out, filt = self._new(False), self._new(False)
+        self._filter_smoother(self.prob, models, sigma, self.ys, models.keep["m0"], models.keep["L0"],
+                              (xm, xL), out, filt)
+        d = (out[0] - xm).abs().amax(1).double().cpu().numpy()
+        return np.maximum.reduceat(d, self.prob.state_offsets_host[:-1])
+
     def _dual_models(self, vals, tans):
         dual = {k: _dual(to_dev(vals[k], self.dtype, self.dev), to_dev(tans[k], self.dtype, self.dev))
                 for k in KEYS}

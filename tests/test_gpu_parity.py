@@ -20,6 +20,7 @@ if not torch.cuda.is_available():  # pragma: no cover
 import parsmooth  # noqa: E402
 from parsmooth import FunctionalModel, MVNSqrt, MVNStandard  # noqa: E402
 from parsmooth import models as M  # noqa: E402
+from parsmooth import _native as N  # noqa: E402
 from parsmooth.linearization import cubature, extended, gauss_hermite, unscented  # noqa: E402
 from parsmooth.parallel import _operators as ops  # noqa: E402
 from parsmooth.parallel._filtering import _sqrt_associative_params_one, _standard_associative_params_one  # noqa: E402
@@ -388,7 +389,7 @@ def test_every_schedule_matches_oracle(chunk):
     schedule the autotuner may pick, against the oracle per sequence."""
     lengths = [300, 17, 1, 2500, 64]
     to, oo = _bearings_setup(2, seed=0)[1]
-    for k in (2, 4, 8):
+    for k in (2, 4, 8, 2 | N.SCAN_TEAM, 8 | N.SCAN_TEAM):
         r, obs, m0, nom = _schedule_runner(lengths, 3, chunk, k)
         assert r.chunk_size() == chunk
         out = r.prob.new_traj()
@@ -407,7 +408,7 @@ def test_deep_scan_schedules_agree():
     chunk 2/K 2) give the same pass to FP64 rounding."""
     T = 300000
     outs = []
-    for chunk, k in ((32, 8), (4, 4), (2, 2)):
+    for chunk, k in ((32, 8), (4, 4), (2, 2), (4, 4 | N.SCAN_TEAM), (2, 2 | N.SCAN_TEAM)):
         r, _, _, nom = _schedule_runner([T], 4, chunk, k)
         out = r.prob.new_traj()
         r.pass_once((t64(nom[0]), t64(nom[1])), out)
@@ -507,3 +508,37 @@ def test_fp32_fallback_recomputes_in_fp64():
     print("fp32 mean errors (no fallback, all recomputed):", errs, "fallback counts:", counts)
     assert counts[0] == 0 and counts[1] > 0
     assert errs[1] <= 1.5 * errs[0]
+
+
+def test_fp32_cubature_pass_matches_oracle():
+    """FP32 cubature pass on the bearings model at a tight nominal (chol 0.1 I).
+    In FP32 the rank-one downdates of the transition's statistical linear
+    regression would cancel (chol Q ~ 6e-5 against a sigma-point spread ~ 0.1);
+    the FP32 path runs the regression in FP64, and the pass matches the FP64
+    oracle on the same FP32-rounded inputs."""
+    from parsmooth.batch import IteratedSmoother
+    from parsmooth.data import bearings_data
+    T = 1000
+    _, truth, ys = bearings_data(np.array([0.1, 0.2, 1.0, 0.0]), 0.01, 0.5, T, (-1.5, 0.5), (1.0, 1.0),
+                                 random_state=1)
+    r32 = lambda a: np.asarray(a, dtype=np.float32).astype(np.float64)
+    _, (to64, oo64), (td64, od64) = _bearings_setup(2)
+    cQ, cR = r32(npy(td64.mvn.chol)), r32(npy(od64.mvn.chol))
+    to = O.FunctionalModel(O.CoordinatedTurn(float(np.float32(0.01))), O.MVNSqrt(np.zeros(5), cQ))
+    oo = O.FunctionalModel(O.Bearings((-1.5, 0.5), (1.0, 1.0)), O.MVNSqrt(np.zeros(2), cR))
+    nom_m = r32(truth)
+    nom_c = r32(np.repeat(0.1 * np.eye(5)[None], T + 1, 0))
+    m0 = r32([0.1, 0.2, 1.0, 0.0, 0.0])
+    ys = r32(ys)
+    ref = O.filter_smoother(ys, O.MVNSqrt(m0, np.eye(5)), to, oo, O.cubature, O.MVNSqrt(nom_m, nom_c), True)
+    t32 = lambda a: torch.as_tensor(np.asarray(a), dtype=torch.float32, device=DEV)
+    td = FunctionalModel(td64.function, MVNSqrt(t32(np.zeros(5)), t32(cQ)))
+    od = FunctionalModel(od64.function, MVNSqrt(t32(np.zeros(2)), t32(cR)))
+    r = IteratedSmoother([T], td, od, cubature, dtype=torch.float32)
+    r.set_data(t32(ys), t32(m0[None]), t32(np.eye(5)[None]))
+    out = r.prob.new_traj()
+    r.pass_once((t32(nom_m), t32(nom_c)), out)
+    torch.cuda.synchronize()
+    assert torch.isfinite(out[0]).all() and torch.isfinite(out[1]).all()
+    close(npy(out[0]), ref.mean, tol=TOL32, what="fp32 cubature mean")
+    close(gram(npy(out[1])), gram(ref.chol), tol=TOL32_COV, what="fp32 cubature cov")
