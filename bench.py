@@ -43,6 +43,30 @@ PEAK_FP32 = 157.3e12
 PEAK_HBM = 8.0e12
 
 
+# rocprofv3 names of the kernels timed under a launch label (FP64, nx=5, ny=2)
+PMC_NAMES = {
+    "k_fscan_up_l0": "psm::k_scan_up<psm::FAgg<double, 5>, double, true, 64>",
+    "k_sscan_up_l0": "psm::k_scan_up<psm::SAgg<double, 5>, double, true, 64>",
+    "k_filter_reduce": "psm::k_filter_reduce<double, 5, 2, true>",
+    "k_filter_apply": "psm::k_filter_apply<double, 5, 2, true>",
+    "k_smoother_reduce": "psm::k_smoother_reduce<double, 5, 2, true>",
+    "k_smoother_apply": "psm::k_smoother_apply<double, 5, true>",
+}
+
+
+def measured_traffic(label):
+    """HBM bytes per (largest) dispatch of the kernel, from the FETCH_SIZE /
+    WRITE_SIZE passes of the same command summarised in profiles/ (scaled by a
+    copy kernel of known size); None when no such summary is present."""
+    path = os.path.join(ROOT, "profiles", "pmc_traffic_bearings_ieks_T131072.json")
+    try:
+        with open(path) as fh:
+            k = json.load(fh)["kernels"].get(PMC_NAMES.get(label, ""))
+    except (OSError, ValueError, KeyError):
+        return None
+    return None if k is None else k["hbm_bytes_max_scaled"]
+
+
 def log(msg):
     """Progress line on stderr (the JSON result is the only stdout line)."""
     print(f"[bench {time.strftime('%H:%M:%S')}] {msg}", file=sys.stderr, flush=True)
@@ -425,7 +449,7 @@ def main():
     kflops = fl * units
     achieved = kflops / per_launch_s / 1e12
     roof = {"bound": "mfma", "achieved": achieved, "peak": PEAK_FP64 / 1e12, "unit": "TFLOP/s",
-            "frac": achieved / (PEAK_FP64 / 1e12), "traffic": None,
+            "frac": achieved / (PEAK_FP64 / 1e12), "traffic": measured_traffic(dom),
             "kernel": dom, "kernel_avg_us": per_launch_s * 1e6,
             "note": "FP64 VALU work (no MFMA on <=10x10 blocks); peak = MI355X FP64 vector rate"}
     cpu = None
