@@ -51,6 +51,14 @@ extern "C" {
 #define PSM_MODEL_COS 4
 #define PSM_MODEL_EXP 5
 #define PSM_MODEL_ARCTAN 6
+/* Values evaluated by the caller (any function the caller can evaluate,
+ * e.g. a torch callable), one row per linearization point -- per state for
+ * psm_filter / psm_smoother / psm_filter_smoother (the transition of step
+ * k -> k+1 and the observation of state k both use row k), per point for
+ * psm_linearize.  Row layout: with a sigma table, f at the points
+ * m + L xi_p (npts x nout, p in table order, L the chol of the nominal);
+ * without one, f(m) followed by its Jacobian (nout + nout x nin). */
+#define PSM_MODEL_TABULATED 7
 
 /* status codes */
 #define PSM_OK 0

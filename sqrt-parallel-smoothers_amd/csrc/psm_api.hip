@@ -73,6 +73,9 @@ static int check_model(const psm_model_t* m, int nin, int nout) {
       if (nout != 2 || nin < 2) { set_error("bearings needs ny = 2 and nx >= 2"); return PSM_E_ARG; }
       if (!m->params) { set_error("bearings needs sensor positions"); return PSM_E_ARG; }
       return PSM_OK;
+    case PSM_MODEL_TABULATED:
+      if (!m->params) { set_error("tabulated model needs its value table"); return PSM_E_ARG; }
+      return PSM_OK;
     case PSM_MODEL_SIN: case PSM_MODEL_COS: case PSM_MODEL_EXP: case PSM_MODEL_ARCTAN:
       if (nin != nout) { set_error("elementwise model needs nin == nout"); return PSM_E_ARG; }
       return PSM_OK;

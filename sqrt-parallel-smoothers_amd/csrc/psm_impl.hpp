@@ -209,8 +209,10 @@ struct Impl {
     if (n <= 0) return PSM_OK;
     int use = 0;
     SigmaView<R> sv = sigma_view<R>(sg, &use);
+    ModelView<R> mv = model_view<R>(md);
+    mv.npts = use ? sv.npts : 0;
     hipLaunchKernelGGL((k_linearize_points<R, NIN, NOUT, SQRT>), dim3(grid_for(n, kLaneWG)),
-                       dim3(kLaneWG), 0, st, n, model_view<R>(md), sv, use,
+                       dim3(kLaneWG), 0, st, n, mv, sv, use,
                        static_cast<const R*>(xm), static_cast<const R*>(xL), static_cast<R*>(F),
                        static_cast<R*>(C), static_cast<R*>(b));
     return hip_check("k_linearize_points");
@@ -348,10 +350,13 @@ struct Impl {
                               hipStream_t st) {
     int use = 0;
     SigmaView<R> sv = sigma_view<R>(sg, &use);
+    ModelView<R> trv = model_view<R>(tr);
     ModelView<R> obv = ob ? model_view<R>(ob) : model_view<R>(tr);
+    trv.npts = use ? sv.npts : 0;
+    obv.npts = use ? sv.npts : 0;
     TimedScope ts_("k_linearize_states", st);
     hipLaunchKernelGGL((k_linearize_states<R, NX, NY, SQRT>), dim3(grid_for(pl.Mp, kLaneWG)),
-                       dim3(kLaneWG), 0, st, pl.M, pl.Mp, pl.chunk, model_view<R>(tr), obv,
+                       dim3(kLaneWG), 0, st, pl.M, pl.Mp, pl.chunk, trv, obv,
                        ob ? 1 : 0, sv, use,
                        (const R*)nm, (const R*)nL, (R*)(ws + pl.off_lin));
     return hip_check("k_linearize_states");

@@ -177,7 +177,7 @@ PSM_DEV void linearize_point_w(const ModelView<R>& md, const SigmaView<R>& sp, i
   using W = typename WiderOf<R>::type;
   if constexpr (SQRT && !std::is_same<W, R>::value) {
     if (use_sigma) {
-      const ModelView<W, R> mw{md.kind, md.params, md.q_mean, md.q_chol};
+      const ModelView<W, R> mw{md.kind, md.params, md.q_mean, md.q_chol, md.row, md.pt, md.npts};
       const SigmaView<W, R> sw{sp.npts, sp.xi, sp.wm, sp.wc};
       W mW[NIN], LW[NIN][NIN], FW[NOUT][NIN], CW[NOUT][NOUT], bW[NOUT];
       widen(m, mW);
@@ -203,6 +203,8 @@ __global__ __launch_bounds__(kLaneWG) void k_linearize_states(
   const int64_t r = t - blk * g;
   const int64_t i = ((blk << 6) + (r & 63)) * C + (r >> 6);
   if (i >= M) return;
+  tr.row = i;
+  ob.row = i;
   R m[NX], L[NX][NX];
   ld_aos(nom_m, i, m);
   ld_aos(nom_L, i, L);
@@ -594,6 +596,7 @@ __global__ __launch_bounds__(kLaneWG) void k_linearize_points(
     const R* __restrict__ xL, R* __restrict__ F, R* __restrict__ C, R* __restrict__ b) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
+  md.row = i;
   R m[NIN], L[NIN][NIN], Fo[NOUT][NIN], Co[NOUT][NOUT], bo[NOUT];
   ld_aos(xm, i, m);
   ld_aos(xL, i, L);

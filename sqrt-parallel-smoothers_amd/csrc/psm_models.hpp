@@ -21,7 +21,13 @@ struct ModelView {
   const S* params;   // model parameters (device)
   const S* q_mean;   // additive noise mean (device, nout)
   const S* q_chol;   // noise chol (sqrt form) or covariance (covariance form), nout x nout
+  // PSM_MODEL_TABULATED: the row (linearization point) and sigma point read,
+  // and the number of sigma points per row (0: rows hold f(m) and its Jacobian)
+  int64_t row = 0;
+  int pt = 0;
+  int npts = 0;
   PSM_DEV R par(int i) const { return R(params[i]); }
+  PSM_DEV R par64(int64_t i) const { return R(params[i]); }
   PSM_DEV R qm(int i) const { return R(q_mean[i]); }
   PSM_DEV R qc(int i) const { return R(q_chol[i]); }
 };
@@ -54,6 +60,26 @@ template <typename B> PSM_DEV DualT<B> atan2_(DualT<B> y, DualT<B> x) {
 template <typename R, int NIN, int NOUT, bool JAC, typename S>
 PSM_DEV void model_eval(const ModelView<R, S>& md, const R (&x)[NIN], R (&y)[NOUT], R (&J)[NOUT][NIN]) {
   switch (md.kind) {
+    case PSM_MODEL_TABULATED: {
+      // values evaluated by the caller: per row either f at each sigma point
+      // (npts x NOUT) or f(m) followed by the Jacobian (NOUT + NOUT x NIN)
+      if (md.npts > 0) {
+        const int64_t base = (md.row * md.npts + md.pt) * NOUT;
+#pragma unroll
+        for (int i = 0; i < NOUT; ++i) y[i] = md.par64(base + i);
+      } else {
+        const int64_t base = md.row * (NOUT + NOUT * NIN);
+#pragma unroll
+        for (int i = 0; i < NOUT; ++i) {
+          y[i] = md.par64(base + i);
+          if (JAC) {
+#pragma unroll
+            for (int j = 0; j < NIN; ++j) J[i][j] = md.par64(base + NOUT + i * NIN + j);
+          }
+        }
+      }
+      break;
+    }
     case PSM_MODEL_LINEAR: {
       // x -> A x, A row-major NOUT x NIN
 #pragma unroll
@@ -209,7 +235,9 @@ PSM_DEV void linearize_sigma(const ModelView<R, S>& md, const SigmaView<R, S>& s
       chi[i] = s;
     }
     R f[NOUT];
-    model_value<R, NIN, NOUT>(md, chi, f);
+    ModelView<R, S> mp = md;
+    mp.pt = p;
+    model_value<R, NIN, NOUT>(mp, chi, f);
     const R w = sp.pwm(p);
 #pragma unroll
     for (int i = 0; i < NOUT; ++i) mf[i] = fma(w, f[i], mf[i]);
@@ -249,7 +277,9 @@ PSM_DEV void linearize_sigma(const ModelView<R, S>& md, const SigmaView<R, S>& s
       dx[i] = s - m[i];
     }
     R f[NOUT];
-    model_value<R, NIN, NOUT>(md, chi, f);
+    ModelView<R, S> mp = md;
+    mp.pt = p;
+    model_value<R, NIN, NOUT>(mp, chi, f);
     const R w = sp.pwc(p);
     R d[NOUT];
 #pragma unroll

@@ -10,6 +10,7 @@ import numpy as np
 import torch
 
 from . import _native as N
+from .models import TorchFunction
 from ._base import MVNSqrt, MVNStandard
 from .linearization._methods import Method, model_struct
 
@@ -139,17 +140,46 @@ class Models:
     """ctypes views of the transition / observation models and the sigma table."""
 
     def __init__(self, prob: Problem, trans, obs, method: Method):
-        self.trans, keep_t = model_struct(trans, prob.dtype, prob.dev, prob.sqrt)
-        self.obs = None
+        self.prob = prob
+        self.trans, keep_t = model_struct(trans, prob.dtype, prob.dev, prob.sqrt, nin=prob.nx)
+        self.trans_fn = keep_t[3]
+        self.obs, self.obs_fn = None, None
         keep_o = ()
         if obs is not None:
-            self.obs, keep_o = model_struct(obs, prob.dtype, prob.dev, prob.sqrt)
-            if obs.function.nin != prob.nx or obs.function.nout != prob.ny:
+            self.obs, keep_o = model_struct(obs, prob.dtype, prob.dev, prob.sqrt, nin=prob.nx)
+            self.obs_fn = keep_o[3]
+            if self.obs_fn.nin != prob.nx or self.obs_fn.nout != prob.ny:
                 raise ValueError("observation function dims do not match the problem")
-        if trans.function.nin != prob.nx or trans.function.nout != prob.nx:
+        if self.trans_fn.nin != prob.nx or self.trans_fn.nout != prob.nx:
             raise ValueError("transition function dims do not match the state dimension")
         self.sigma, keep_s = method.sigma(prob.nx, prob.dtype, prob.dev)
         self._keep = (keep_t, keep_o, keep_s)
+        self._tables = {}
+
+    @property
+    def tabulated(self) -> bool:
+        return isinstance(self.trans_fn, TorchFunction) or isinstance(self.obs_fn, TorchFunction)
+
+    def prepare(self, nom_m, nom_L, obs: bool = True):
+        """Evaluate torch-callable models at the linearization points of this
+        nominal trajectory (row i = state i) into persistent tables."""
+        if not self.tabulated:
+            return
+        xi = self._keep[2][0] if self.sigma is not None else None
+        L = nom_L
+        if xi is not None and not self.prob.sqrt:
+            L = torch.linalg.cholesky(nom_L)   # the kernels factor the covariance the same way
+        for name, fn, struct in (("trans", self.trans_fn, self.trans),
+                                 ("obs", self.obs_fn, self.obs if obs else None)):
+            if struct is None or not isinstance(fn, TorchFunction):
+                continue
+            width = (xi.shape[0] * fn.nout) if xi is not None else fn.nout * (1 + fn.nin)
+            buf = self._tables.get(name)
+            if buf is None or buf.shape != (self.prob.n_states, width):
+                buf = torch.empty((self.prob.n_states, width), dtype=self.prob.dtype, device=self.prob.dev)
+                self._tables[name] = buf
+            fn.tabulate(nom_m.reshape(-1, fn.nin), L.reshape(-1, fn.nin, fn.nin), xi, out=buf)
+            struct.params = buf.data_ptr()
 
     def sigma_ref(self):
         return ctypes.byref(self.sigma) if self.sigma is not None else None
@@ -159,6 +189,7 @@ def filter_pass(prob: Problem, mods: Models, ys, m0, L0, nom_m, nom_L, want_ell:
                 out=None):
     out_m, out_L = out if out is not None else prob.new_traj()
     ell = torch.empty(prob.n_seq, dtype=torch.float64, device=prob.dev) if want_ell else None
+    mods.prepare(nom_m, nom_L)
     rc = N.lib().psm_filter(ctypes.byref(prob.struct), ctypes.byref(mods.trans), ctypes.byref(mods.obs),
                             mods.sigma_ref(), N.ptr(ys), N.ptr(m0), N.ptr(L0), N.ptr(nom_m),
                             N.ptr(nom_L), N.ptr(out_m), N.ptr(out_L), N.ptr(ell),
@@ -169,6 +200,7 @@ def filter_pass(prob: Problem, mods: Models, ys, m0, L0, nom_m, nom_L, want_ell:
 
 def smoother_pass(prob: Problem, mods: Models, fm, fL, nom_m, nom_L, out=None):
     out_m, out_L = out if out is not None else prob.new_traj()
+    mods.prepare(nom_m, nom_L, obs=False)
     rc = N.lib().psm_smoother(ctypes.byref(prob.struct), ctypes.byref(mods.trans), mods.sigma_ref(),
                               N.ptr(fm), N.ptr(fL), N.ptr(nom_m), N.ptr(nom_L), N.ptr(out_m),
                               N.ptr(out_L), N.ptr(prob.workspace), prob.ws_bytes, N.stream_ptr())
@@ -180,6 +212,7 @@ def filter_smoother_pass(prob: Problem, mods: Models, ys, m0, L0, nom_m, nom_L, 
                          use_graph: bool = False):
     fm, fL = filt
     om, oL = out
+    mods.prepare(nom_m, nom_L)
     rc = N.lib().psm_filter_smoother(ctypes.byref(prob.struct), ctypes.byref(mods.trans),
                                      ctypes.byref(mods.obs), mods.sigma_ref(), N.ptr(ys), N.ptr(m0),
                                      N.ptr(L0), N.ptr(nom_m), N.ptr(nom_L), N.ptr(fm), N.ptr(fL),

@@ -20,6 +20,12 @@ from ._engine import Models, Problem, filter_pass, filter_smoother_pass, to_dev
 from .linearization._methods import resolve
 
 
+def _noise_dim(model) -> int:
+    """Output dimension of a FunctionalModel: the size of its noise mean."""
+    q = model[1][0]
+    return int(q.numel()) if isinstance(q, torch.Tensor) else int(np.prod(np.shape(q)))
+
+
 class IteratedSmoother:
     """Iterated (IEKS / IPLS) parallel smoother for a ragged batch.
 
@@ -37,8 +43,8 @@ class IteratedSmoother:
         self.dev = N.device()
         self.dtype = dtype
         self.sqrt = sqrt
-        nx = transition_model.function.nin
-        ny = observation_model.function.nout
+        nx = _noise_dim(transition_model)
+        ny = _noise_dim(observation_model)
         self.prob = Problem(nx, ny, lengths, dtype, sqrt, self.dev, chunk)
         self.mods = Models(self.prob, transition_model, observation_model, resolve(linearization_method))
         self.nx, self.ny = nx, ny

@@ -11,7 +11,7 @@ import torch
 
 from .. import _native as N
 from .._base import ConditionalMomentsModel, FunctionalModel, MVNSqrt, MVNStandard, are_inputs_compatible
-from ..models import is_device_function
+from ..models import TorchFunction, as_device_function
 from . import _tables
 
 
@@ -77,27 +77,38 @@ def resolve(method) -> Method:
     return m
 
 
-def model_struct(model: FunctionalModel, dtype, dev, sqrt: bool):
-    """PsmModel for a FunctionalModel plus the tensors it points to."""
+def model_function(model: FunctionalModel, nin: int):
+    """The device function of a FunctionalModel: a parsmooth.models function as
+    is, any other callable wrapped in a TorchFunction (input dimension nin,
+    output dimension that of the noise mean)."""
     if isinstance(model, ConditionalMomentsModel):
         raise NotImplementedError("ConditionalMomentsModel is not supported by the HIP path")
     if not isinstance(model, FunctionalModel):
         raise TypeError("model must be a FunctionalModel")
     f, q = model
-    if not is_device_function(f):
-        raise TypeError(
-            "FunctionalModel.function must be a parsmooth.models function (LinearFunction, "
-            "CoordinatedTurnTransition, BearingsObservation, Elementwise) so the HIP kernels can "
-            f"evaluate it; got {f!r}")
+    nout = int(np.prod(np.shape(q[0]))) if not isinstance(q[0], torch.Tensor) else int(q[0].numel())
+    return as_device_function(f, nin, nout)
+
+
+def model_struct(model: FunctionalModel, dtype, dev, sqrt: bool, nin: int = None):
+    """PsmModel for a FunctionalModel plus the tensors it points to (and the
+    device function).  For a TorchFunction the value table is attached per call
+    (``params`` is set by whoever tabulates)."""
+    f = model_function(model, nin if nin is not None else getattr(model.function, "nin", 0))
+    q = model[1]
     if sqrt != isinstance(q, MVNSqrt):
         raise TypeError("the model noise must be MVNSqrt in square-root form and MVNStandard otherwise")
     qm = _to_dev(q[0], dtype, dev).reshape(-1)
     qc = _to_dev(q[1], dtype, dev).reshape(f.nout, f.nout)
     if qm.numel() != f.nout:
         raise ValueError(f"noise mean has {qm.numel()} entries, the function outputs {f.nout}")
-    params = f.device_params(dtype, dev)
-    s = N.PsmModel(f.kind, f.nin, f.nout, 0, params.data_ptr(), qm.data_ptr(), qc.data_ptr())
-    return s, (params, qm, qc)
+    if isinstance(f, TorchFunction):
+        params = None
+        s = N.PsmModel(f.kind, f.nin, f.nout, 0, 0, qm.data_ptr(), qc.data_ptr())
+    else:
+        params = f.device_params(dtype, dev)
+        s = N.PsmModel(f.kind, f.nin, f.nout, 0, params.data_ptr(), qm.data_ptr(), qc.data_ptr())
+    return s, (params, qm, qc, f)
 
 
 def linearize(model, x, method: Method):
@@ -115,7 +126,8 @@ def linearize(model, x, method: Method):
     n = m.shape[-1]
     batch = m.shape[:-1]
     L = _to_dev(x[1], dtype, dev).reshape(batch + (n, n))
-    ms, st = model_struct(model, dtype, dev, sqrt)
+    ms, st = model_struct(model, dtype, dev, sqrt, nin=n)
+    f = st[3]
     if f.nin != n:
         raise ValueError(f"function expects dimension {f.nin}, x has {n}")
     cnt = int(np.prod(batch)) if len(batch) else 1
@@ -123,6 +135,14 @@ def linearize(model, x, method: Method):
     C = torch.empty(batch + (f.nout, f.nout), dtype=dtype, device=dev)
     b = torch.empty(batch + (f.nout,), dtype=dtype, device=dev)
     sig, _keep = method.sigma(n, dtype, dev)
+    table = None
+    if isinstance(f, TorchFunction):
+        mm_ = m.reshape(cnt, n)
+        LL = L.reshape(cnt, n, n)
+        if sig is not None and not sqrt:
+            LL = torch.linalg.cholesky(LL)   # the kernel factors the covariance the same way
+        table = f.tabulate(mm_, LL, _keep[0] if sig is not None else None)
+        ms.params = table.data_ptr()
     rc = N.lib().psm_linearize(N.dtype_code(dtype), N.FORM_SQRT if sqrt else N.FORM_STD,
                                ctypes.byref(ms), ctypes.byref(sig) if sig is not None else None,
                                cnt, N.ptr(m), N.ptr(L), N.ptr(F), N.ptr(C), N.ptr(b), N.stream_ptr())

@@ -10,6 +10,12 @@ for convenience, and carries the identifiers the kernels need.
 * ``CoordinatedTurnTransition(dt)``          (tests/bearings/bearings_utils.py:7-44)
 * ``BearingsObservation(s1, s2)``            (tests/bearings/bearings_utils.py:47-69)
 * ``Elementwise("sin"|"cos"|"exp"|"arctan", dim)`` (tests/test_linearization.py:157)
+* ``TorchFunction(fn, nin, nout)``: any torch-callable model.  The host
+  evaluates ``fn`` (and, for the extended linearization, its Jacobian with
+  ``torch.func``) at the points each linearization needs and hands the values
+  to the kernels as a table; the regression, the elements and the scans stay
+  on the device.  A plain callable in ``FunctionalModel.function`` is wrapped
+  in one automatically.
 
 ``make_parameters`` mirrors tests/bearings/bearings_utils.py:72-115 and
 notebooks/bearing_data.py:93-134.
@@ -165,3 +171,62 @@ def make_parameters(qc, qw, r, dt, s1, s2):
 
 def is_device_function(f) -> bool:
     return isinstance(f, DeviceFunction)
+
+
+class TorchFunction(DeviceFunction):
+    """A model function evaluated by torch on the host side of the boundary.
+
+    ``fn`` maps a tensor of shape (..., nin) to (..., nout) with torch
+    operations (for the extended linearization it must also work under
+    ``torch.func.vmap`` / ``jacfwd``).  ``tabulate`` evaluates it at the
+    linearization points: f(m) and the Jacobian per point, or f at every
+    sigma point m + L xi_p."""
+
+    kind = N.MODEL_TABULATED
+
+    def __init__(self, fn, nin: int, nout: int):
+        super().__init__()
+        if isinstance(fn, DeviceFunction):
+            raise TypeError("already a device function")
+        self.fn = fn
+        self.nin = int(nin)
+        self.nout = int(nout)
+
+    def __call__(self, x):
+        return self.fn(x)
+
+    def _eval(self, x):
+        y = self.fn(x)
+        if not isinstance(y, torch.Tensor):
+            y = torch.as_tensor(y, dtype=x.dtype, device=x.device)
+        return y.reshape(x.shape[:-1] + (self.nout,))
+
+    def tabulate(self, m: torch.Tensor, L: torch.Tensor, xi, out: torch.Tensor = None):
+        """Rows of the value table at points m (n, nin) with factors L (n, nin, nin):
+        xi None -> [f(m), J(m)] (n, nout + nout nin); else f(m + L xi_p) (n, npts nout).
+        ``out`` (n, row width), when given, receives the rows in place."""
+        n = m.shape[0]
+        if xi is None:
+            f1 = lambda v: self._eval(v.reshape(1, self.nin)).reshape(self.nout)
+            y = self._eval(m)
+            J = torch.func.vmap(torch.func.jacfwd(f1))(m)
+            rows = torch.cat([y.reshape(n, self.nout), J.reshape(n, self.nout * self.nin)], dim=1)
+        else:
+            xi = xi.to(dtype=m.dtype, device=m.device)
+            chi = m[:, None, :] + torch.einsum("nij,pj->npi", L, xi)
+            rows = self._eval(chi.reshape(-1, self.nin)).reshape(n, xi.shape[0] * self.nout)
+        rows = rows.to(m.dtype)
+        if out is None:
+            return rows.contiguous()
+        out.copy_(rows)
+        return out
+
+
+def as_device_function(f, nin: int, nout: int) -> DeviceFunction:
+    """``f`` itself when the kernels evaluate it, else ``f`` wrapped in a
+    TorchFunction of the given dimensions."""
+    if isinstance(f, DeviceFunction):
+        return f
+    if not callable(f):
+        raise TypeError(f"model function must be callable, got {f!r}")
+    return TorchFunction(f, nin, nout)
