@@ -202,6 +202,20 @@ int psm_filter_smoother(const psm_problem_t* prob, const psm_model_t* trans,
                         void* filt_m, void* filt_L, void* out_m, void* out_L, void* workspace,
                         size_t workspace_bytes, int32_t use_graph, void* stream);
 
+/* Pathwise samples of the smoothing distribution at a nominal trajectory.
+ * Replaces parsmooth/_pathwise_sampler.py:13-38 (parallel sampling, reached
+ * from parsmooth/methods.py:79-91) with its elements :64-127.  From the
+ * filtered trajectory (filt_m, filt_L) and the transition linearized at the
+ * nominal, every sample path follows x_k = E_k x_{k+1} + g_k + D_k eps_k
+ * backwards from x_T = m_T + L_T eps_T (smoother elements; D_k's Cholesky
+ * factor in covariance form).  eps: standard normals drawn by the caller,
+ * (n_states, n_samples, nx); out: samples, same shape.  PSM_F32 / PSM_F64. */
+size_t psm_sample_workspace_bytes(const psm_problem_t* prob, int32_t n_samples);
+int psm_sample(const psm_problem_t* prob, const psm_model_t* trans, const psm_sigma_t* sigma,
+               const void* filt_m, const void* filt_L, const void* nom_m, const void* nom_L,
+               const void* eps, int32_t n_samples, void* out, void* workspace,
+               size_t workspace_bytes, void* stream);
+
 /* mean((a - b)^2) over n values, accumulated in FP64 into *out (device).
  * The default stopping criterion of parsmooth/methods.py:50-51. */
 int psm_mean_sq_diff(int32_t dtype, int64_t n, const void* a, const void* b, double* out,

@@ -676,6 +676,35 @@ def std_smoother_element(F, Q, bt, m, P):
     return g, E, Lc
 
 
+def sampling(eps, trans, filt, method, nominal):
+    """Pathwise samples: parsmooth/_pathwise_sampler.py:41-61 (the sequential
+    recursion x_k = gain_k x_{k+1} + inc_k) with the gains and increments of
+    :64-127, which are the smoother elements of parallel/_smoothing.py:60-85
+    (gain = E, increment mean = g, increment chol = D, or chol(D) in covariance
+    form).  eps (T+1, S, nx): eps[k] drives state k, eps[T] the last state (the
+    reference's eps[0]).  Returns (T+1, S, nx)."""
+    ms = np.asarray(filt[0], dtype=np.float64)
+    Ps = np.asarray(filt[1], dtype=np.float64)
+    sqrt = isinstance(filt, MVNSqrt)
+    cls = MVNSqrt if sqrt else MVNStandard
+    F, cq, bt = linearize_many(method, trans, cls(nominal[0][:-1], nominal[1][:-1]))
+    if sqrt:
+        g, E, D = sqrt_smoother_element(F, cq, bt, ms[:-1], Ps[:-1])
+        Lk, LT = D, Ps[-1]
+    else:
+        g, E, D = std_smoother_element(F, cq, bt, ms[:-1], Ps[:-1])
+        Lk, LT = np.linalg.cholesky(D), np.linalg.cholesky(Ps[-1])
+    eps = np.asarray(eps, dtype=np.float64)
+    T = ms.shape[0] - 1
+    out = np.empty(eps.shape)
+    x = ms[-1] + eps[T] @ LT.T
+    out[T] = x
+    for k in range(T - 1, -1, -1):
+        x = x @ E[k].T + g[k] + eps[k] @ Lk[k].T
+        out[k] = x
+    return out
+
+
 def par_smoothing(trans, filt, method, nominal=None, scan=associative_scan):
     """parsmooth/parallel/_smoothing.py:14-57."""
     ms = np.asarray(filt[0], dtype=np.float64)

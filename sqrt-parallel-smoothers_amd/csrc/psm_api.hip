@@ -361,6 +361,36 @@ int psm_filter_smoother(const psm_problem_t* p, const psm_model_t* trans, const 
   return PSM_OK;
 }
 
+size_t psm_sample_workspace_bytes(const psm_problem_t* p, int32_t n_samples) {
+  if (check_problem(p)) return 0;
+  if (n_samples < 1) { set_error("n_samples must be >= 1"); return 0; }
+  const OpsTable* t = need(p->dtype, p->form, p->nx, p->ny);
+  if (!t) return 0;
+  return t->sample_workspace_bytes(p, n_samples);
+}
+
+int psm_sample(const psm_problem_t* p, const psm_model_t* trans, const psm_sigma_t* sigma,
+               const void* filt_m, const void* filt_L, const void* nom_m, const void* nom_L,
+               const void* eps, int32_t n_samples, void* out, void* ws, size_t wsb,
+               void* stream) {
+  int rc = check_problem(p);
+  if (rc) return rc;
+  if (p->dtype != PSM_F32 && p->dtype != PSM_F64) {
+    set_error("psm_sample: dtype must be PSM_F32 or PSM_F64");
+    return PSM_E_ARG;
+  }
+  if (n_samples < 1) { set_error("n_samples must be >= 1"); return PSM_E_ARG; }
+  if ((rc = check_model(trans, p->nx, p->nx))) return rc;
+  if (!filt_m || !filt_L || !nom_m || !nom_L || !eps || !out) {
+    set_error("psm_sample: null array");
+    return PSM_E_ARG;
+  }
+  const OpsTable* t = need(p->dtype, p->form, p->nx, p->ny);
+  if (!t) return PSM_E_UNSUPPORTED;
+  return t->sample(p, trans, sigma, filt_m, filt_L, nom_m, nom_L, eps, n_samples, out, ws, wsb,
+                   (hipStream_t)stream);
+}
+
 int psm_mean_sq_diff(int32_t dtype, int64_t n, const void* a, const void* b, double* out,
                      void* stream) {
   hipStream_t st = (hipStream_t)stream;

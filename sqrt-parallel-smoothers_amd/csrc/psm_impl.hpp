@@ -6,6 +6,7 @@
 #include <string>
 
 #include "psm_kernels.hpp"
+#include "psm_sample.hpp"
 
 namespace psm {
 
@@ -171,6 +172,10 @@ struct OpsTable {
   int (*smoother)(const psm_problem_t*, const psm_model_t*, const psm_sigma_t*, const void*,
                   const void*, const void*, const void*, void*, void*, void*, size_t, hipStream_t,
                   int lin_done);
+  size_t (*sample_workspace_bytes)(const psm_problem_t*, int);
+  int (*sample)(const psm_problem_t*, const psm_model_t*, const psm_sigma_t*, const void*,
+                const void*, const void*, const void*, const void*, int, void*, void*, size_t,
+                hipStream_t);
 };
 
 template <typename R, int NX, int NY, bool SQRT>
@@ -445,10 +450,68 @@ struct Impl {
     return hip_check("k_smoother_apply");
   }
 
+  // ---------------------------------------------------------------- sampling
+  struct SamplePlan {
+    int64_t Cs, nch;
+    size_t off_P, off_o, off_xb, total;
+  };
+  static SamplePlan sample_plan(const Plan& pl, int S) {
+    SamplePlan sp;
+    sp.Cs = sample_chunk(pl.M);
+    sp.nch = (pl.M + sp.Cs - 1) / sp.Cs;
+    size_t off = pl.total;
+    sp.off_P = off;
+    off = align_up(off + (size_t)sp.nch * NX * NX * sizeof(R));
+    sp.off_o = off;
+    off = align_up(off + (size_t)sp.nch * S * NX * sizeof(R));
+    sp.off_xb = off;
+    off = align_up(off + (size_t)sp.nch * S * NX * sizeof(R));
+    sp.total = off;
+    return sp;
+  }
+  static size_t sample_workspace_bytes(const psm_problem_t* p, int S) {
+    return sample_plan(plan(p), S).total;
+  }
+  static int sample(const psm_problem_t* p, const psm_model_t* tr, const psm_sigma_t* sg,
+                    const void* fm, const void* fL, const void* nm, const void* nL, const void* eps,
+                    int S, void* out, void* wsv, size_t wsb, hipStream_t st) {
+    const Plan pl = plan(p);
+    const SamplePlan sp = sample_plan(pl, S);
+    if (wsb < sp.total) {
+      set_error("workspace too small");
+      return PSM_E_WORKSPACE;
+    }
+    if (pl.M <= 0 || S <= 0) return PSM_OK;
+    char* ws = (char*)wsv;
+    int rc = linearize_states(pl, tr, nullptr, sg, nm, nL, ws, st);
+    if (rc) return rc;
+    const Fallback fb{p->fallback_ratio, p->fallback_count};
+    // smoother elements at every state (the chunk aggregates it also forms are not used)
+    hipLaunchKernelGGL((k_smoother_reduce<R, NX, NY, SQRT>), dim3(grid_for(pl.nchunks, kLaneWG)),
+                       dim3(kLaneWG), 0, st, pl.M, pl.Mp, pl.chunk, pl.nchunks, p->state_offsets,
+                       p->n_seq, (const R*)(ws + pl.off_lin), (const R*)fm, (const R*)fL,
+                       (R*)(ws + pl.off_selem), (R*)(ws + pl.lv[0].off_in), level_map(pl, 0, 1), fb);
+    if ((rc = hip_check("k_smoother_reduce"))) return rc;
+    const int64_t lanes = sp.nch * S;
+    hipLaunchKernelGGL((k_sample_reduce<R, NX, SQRT>), dim3(grid_for(lanes, kLaneWG)), dim3(kLaneWG),
+                       0, st, pl.M, pl.Mp, pl.chunk, sp.Cs, sp.nch, S, (const R*)(ws + pl.off_selem),
+                       (const R*)eps, (R*)(ws + sp.off_P), (R*)(ws + sp.off_o));
+    if ((rc = hip_check("k_sample_reduce"))) return rc;
+    hipLaunchKernelGGL((k_sample_bounds<R, NX>), dim3(grid_for(S, kLaneWG)), dim3(kLaneWG), 0, st,
+                       sp.nch, S, (const R*)(ws + sp.off_P), (const R*)(ws + sp.off_o),
+                       (R*)(ws + sp.off_xb));
+    if ((rc = hip_check("k_sample_bounds"))) return rc;
+    hipLaunchKernelGGL((k_sample_apply<R, NX, SQRT>), dim3(grid_for(lanes, kLaneWG)), dim3(kLaneWG),
+                       0, st, pl.M, pl.Mp, pl.chunk, sp.Cs, sp.nch, S, (const R*)(ws + pl.off_selem),
+                       (const R*)eps, (const R*)(ws + sp.off_xb), (R*)out);
+    return hip_check("k_sample_apply");
+  }
+
   static const OpsTable* table() {
     static const OpsTable t = {&linearize_trans, &linearize_obs, &filter_elements,
                                &filter_combine,  &smoother_elements, &smoother_combine,
-                               &workspace_bytes, &filter,          &smoother};
+                               &workspace_bytes, &filter,          &smoother,
+                               &sample_workspace_bytes, &sample};
     return &t;
   }
 };

@@ -36,6 +36,7 @@ EXPORTS = (
     "psm_filter_combine", "psm_smoother_elements", "psm_smoother_combine", "psm_workspace_bytes",
     "psm_filter", "psm_smoother", "psm_filter_smoother", "psm_mean_sq_diff", "psm_clear_graphs",
     "psm_timing_enable", "psm_timing_report", "psm_chunk_size", "psm_autotune",
+    "psm_sample_workspace_bytes", "psm_sample",
 )
 
 
@@ -88,6 +89,9 @@ _SIGS = {
     "psm_timing_enable": (None, [_I32]),
     "psm_chunk_size": (ctypes.c_int, [ctypes.POINTER(PsmProblem)]),
     "psm_timing_report": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_size_t]),
+    "psm_sample_workspace_bytes": (ctypes.c_size_t, [ctypes.POINTER(PsmProblem), _I32]),
+    "psm_sample": (ctypes.c_int, [ctypes.POINTER(PsmProblem), ctypes.POINTER(PsmModel),
+                                  ctypes.POINTER(PsmSigma)] + [_P] * 5 + [_I32, _P, _P, ctypes.c_size_t, _P]),
     "psm_autotune": (ctypes.c_int, [ctypes.POINTER(PsmProblem), ctypes.POINTER(PsmModel),
                                     ctypes.POINTER(PsmModel), ctypes.POINTER(PsmSigma)]
                      + [_P] * 10 + [ctypes.c_size_t, _P, ctypes.POINTER(_I32), ctypes.POINTER(_I32),

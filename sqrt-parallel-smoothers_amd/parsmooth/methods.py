@@ -20,6 +20,7 @@ from ._engine import Models, Problem, compute_dtype, filter_pass, filter_smoothe
 from .linearization._methods import resolve
 from .parallel._filtering import filtering as _par_filtering
 from .parallel._smoothing import smoothing as _par_smoothing
+from .sampling import sampling  # noqa: F401  (parsmooth/methods.py:79-91)
 
 
 def _need_parallel(parallel):
@@ -111,8 +112,3 @@ def iterated_smoothing(observations, x0: Union[MVNSqrt, MVNStandard], transition
     if return_loglikelihood:
         return cur, it.loglikelihood(tuple(cur))
     return cur
-
-
-def sampling(*_args, **_kwargs):
-    """Pathwise sampling is not implemented on the HIP device."""
-    raise NotImplementedError("sampling is not implemented by the HIP path")

@@ -376,3 +376,43 @@ def test_c_restatement_linear_closed_form():
     np.testing.assert_allclose(m, sm, atol=1e-8)
     np.testing.assert_allclose(L @ np.swapaxes(L, 1, 2), sP, atol=1e-8)
     assert e == pytest.approx(ell, rel=1e-9)
+
+
+# ---------------------------------------------------------------- pathwise sampler
+def _linear_smoothing_setup(seed, T=25):
+    np.random.seed(seed)
+    x0, cx0, F, Q, cholQ, b, _ = O.get_system(2, 2)
+    _, _, H, R, cholR, c, _ = O.get_system(2, 1)
+    F = 0.9 * F / max(1.0, np.abs(np.linalg.eigvals(F)).max())
+    rng = np.random.RandomState(seed)
+    ys = rng.randn(T, 1)
+    tq = O.FunctionalModel(O.Linear(F), O.MVNSqrt(b, cholQ))
+    to = O.FunctionalModel(O.Linear(H), O.MVNSqrt(c, cholR))
+    filt = O.filtering(ys, cx0, tq, to, O.extended, None, True)
+    smooth = O.smoothing(tq, filt, O.extended, None, True)
+    return tq, filt, smooth
+
+
+def test_sampler_moments_are_the_smoothing_marginals():
+    """tests/test_sampler.py:27-67: sample means and covariances of the paths
+    match the smoothed marginals (here within 5 standard errors / 5 %)."""
+    tq, filt, smooth = _linear_smoothing_setup(3)
+    S = 40000
+    eps = np.random.RandomState(7).randn(filt[0].shape[0], S, 2)
+    xs = O.sampling(eps, tq, filt, O.extended, smooth)
+    P = smooth.chol @ np.swapaxes(smooth.chol, -1, -2)
+    se = np.sqrt(np.diagonal(P, axis1=1, axis2=2) / S)
+    assert np.all(np.abs(xs.mean(1) - smooth.mean) <= 5 * se)
+    C = np.einsum("tsi,tsj->tij", xs - xs.mean(1, keepdims=True), xs - xs.mean(1, keepdims=True)) / (S - 1)
+    np.testing.assert_allclose(C, P, rtol=0.05, atol=0.05 * np.abs(P).max())
+
+
+def test_sampler_sqrt_equals_std_for_the_same_normals():
+    tq, filt, smooth = _linear_smoothing_setup(11)
+    eps = np.random.RandomState(1).randn(filt[0].shape[0], 16, 2)
+    xs = O.sampling(eps, tq, filt, O.extended, smooth)
+    gram = lambda L: L @ np.swapaxes(L, -1, -2)
+    tq_std = O.FunctionalModel(tq.function, O.MVNStandard(tq.mvn.mean, gram(tq.mvn.chol)))
+    xs2 = O.sampling(eps, tq_std, O.MVNStandard(filt.mean, gram(filt.chol)), O.extended,
+                     O.MVNStandard(smooth.mean, gram(smooth.chol)))
+    np.testing.assert_allclose(xs2, xs, rtol=1e-9, atol=1e-9)
