@@ -59,6 +59,12 @@ extern "C" {
  * m + L xi_p (npts x nout, p in table order, L the chol of the nominal);
  * without one, f(m) followed by its Jacobian (nout + nout x nin). */
 #define PSM_MODEL_TABULATED 7
+/* Conditional-moments model (E[y|x], chol or covariance of y|x) evaluated by
+ * the caller, as PSM_MODEL_TABULATED with each value block followed by the
+ * conditional factor (nout x nout): with a sigma table npts x (nout + nout^2),
+ * without one nout + nout x nin + nout^2 (mean, Jacobian, factor at the
+ * mean).  q_mean / q_chol must point to zeros (no additive noise). */
+#define PSM_MODEL_CONDITIONAL 8
 
 /* status codes */
 #define PSM_OK 0

@@ -57,6 +57,14 @@ class FunctionalModel(NamedTuple):
     mvn: object
 
 
+class ConditionalMomentsModel(NamedTuple):
+    """parsmooth/_base.py:20-22.  ``conditional_mean`` is callable on batched
+    points (..., n) with a ``jac`` method (for the extended linearization);
+    ``conditional_covariance_or_cholesky`` maps (..., n) to (..., d, d)."""
+    conditional_mean: object
+    conditional_covariance_or_cholesky: object
+
+
 def _T(x):
     return np.swapaxes(x, -1, -2)
 
@@ -268,7 +276,14 @@ def get_mvnsqrt(x):
 
 
 def extended(model, x):
-    """parsmooth/linearization/_extended.py:9-40,59-70 (FunctionalModel only)."""
+    """parsmooth/linearization/_extended.py:9-40,59-70; conditional moments
+    :43-56 (F = jacobian of the conditional mean, the conditional factor at the
+    mean, b = E[y|m] - F m)."""
+    if isinstance(model, ConditionalMomentsModel):
+        m = np.asarray(x.mean, dtype=np.float64)
+        F = model.conditional_mean.jac(m)
+        b = model.conditional_mean(m) - np.einsum("...ij,...j->...i", F, m)
+        return F, np.asarray(model.conditional_covariance_or_cholesky(m), np.float64), b
     f, q = model
     m = np.asarray(x.mean, dtype=np.float64)
     F = f.jac(m)
@@ -356,8 +371,37 @@ def cholesky_update_batch(chol, v, multiplier):
     return out
 
 
+def _sigma_point_conditional(model, x, table: SigmaTable):
+    """parsmooth/linearization/_sigma_points.py:25-60 (conditional moments)."""
+    xs = get_mvnsqrt(x)
+    m, L = np.asarray(xs.mean, np.float64), np.asarray(xs.chol, np.float64)
+    pts = m[..., None, :] + table.xi @ _T(L)
+    fp = model.conditional_mean(pts)
+    mf = np.einsum("p,...pd->...d", table.wm, fp)
+    df = fp - mf[..., None, :]
+    Psi = np.einsum("...pn,p,...pd->...nd", pts - m[..., None, :], table.wc, df)
+    F = _T(np.linalg.solve(_T(L), np.linalg.solve(L, Psi)))
+    b = mf - _mv(F, m)
+    V = np.asarray(model.conditional_covariance_or_cholesky(pts), np.float64)   # (..., npts, d, d)
+    if isinstance(x, MVNSqrt):
+        sqrt_phi = _T(np.sqrt(table.wc)[:, None] * df)                           # (..., d, npts)
+        temp = np.sqrt(table.wc)[:, None, None] * V                              # (..., npts, d, d)
+        temp = np.swapaxes(temp, -3, -2).reshape(temp.shape[:-3] + (temp.shape[-2], -1))
+        C = tria(np.concatenate([sqrt_phi, temp], -1))
+        FL = F @ L
+        for j in range(FL.shape[-1]):
+            C = cholesky_update_batch(C, FL[..., :, j], -1.0)
+        return F, C, b
+    Phi = np.einsum("...pd,p,...pe->...de", df, table.wc, df)
+    v_f = np.einsum("p,...pde->...de", table.wc, V)
+    temp = F @ L
+    return F, Phi - temp @ _T(temp) + v_f, b
+
+
 def sigma_point_linearize(model, x, table: SigmaTable):
     """parsmooth/linearization/_sigma_points.py:63-100 (leading batch axes allowed)."""
+    if isinstance(model, ConditionalMomentsModel):
+        return _sigma_point_conditional(model, x, table)
     f, q = model
     xs = get_mvnsqrt(x)
     m, L = np.asarray(xs.mean, np.float64), np.asarray(xs.chol, np.float64)

@@ -74,6 +74,7 @@ static int check_model(const psm_model_t* m, int nin, int nout) {
       if (!m->params) { set_error("bearings needs sensor positions"); return PSM_E_ARG; }
       return PSM_OK;
     case PSM_MODEL_TABULATED:
+    case PSM_MODEL_CONDITIONAL:
       if (!m->params) { set_error("tabulated model needs its value table"); return PSM_E_ARG; }
       return PSM_OK;
     case PSM_MODEL_SIN: case PSM_MODEL_COS: case PSM_MODEL_EXP: case PSM_MODEL_ARCTAN:

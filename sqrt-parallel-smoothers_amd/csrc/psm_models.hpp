@@ -60,15 +60,19 @@ template <typename B> PSM_DEV DualT<B> atan2_(DualT<B> y, DualT<B> x) {
 template <typename R, int NIN, int NOUT, bool JAC, typename S>
 PSM_DEV void model_eval(const ModelView<R, S>& md, const R (&x)[NIN], R (&y)[NOUT], R (&J)[NOUT][NIN]) {
   switch (md.kind) {
-    case PSM_MODEL_TABULATED: {
+    case PSM_MODEL_TABULATED:
+    case PSM_MODEL_CONDITIONAL: {
       // values evaluated by the caller: per row either f at each sigma point
-      // (npts x NOUT) or f(m) followed by the Jacobian (NOUT + NOUT x NIN)
+      // (npts x NOUT) or f(m) followed by the Jacobian (NOUT + NOUT x NIN);
+      // a conditional-moments model adds its conditional factor (NOUT x NOUT)
+      // after each value block
+      const int V = md.kind == PSM_MODEL_CONDITIONAL ? NOUT * NOUT : 0;
       if (md.npts > 0) {
-        const int64_t base = (md.row * md.npts + md.pt) * NOUT;
+        const int64_t base = (md.row * md.npts + md.pt) * (NOUT + V);
 #pragma unroll
         for (int i = 0; i < NOUT; ++i) y[i] = md.par64(base + i);
       } else {
-        const int64_t base = md.row * (NOUT + NOUT * NIN);
+        const int64_t base = md.row * (NOUT + NOUT * NIN + V);
 #pragma unroll
         for (int i = 0; i < NOUT; ++i) {
           y[i] = md.par64(base + i);
@@ -170,6 +174,19 @@ PSM_DEV void model_eval(const ModelView<R, S>& md, const R (&x)[NIN], R (&y)[NOU
   }
 }
 
+// Conditional factor (chol in square-root form, covariance otherwise) of a
+// PSM_MODEL_CONDITIONAL row: at the sigma point md.pt, or at the mean.
+template <typename R, int NIN, int NOUT, typename S>
+PSM_DEV void model_cond_factor(const ModelView<R, S>& md, R (&V)[NOUT][NOUT]) {
+  constexpr int W = NOUT * NOUT;
+  const int64_t base = md.npts > 0 ? (md.row * md.npts + md.pt) * (NOUT + W) + NOUT
+                                   : md.row * (NOUT + NOUT * NIN + W) + NOUT + NOUT * NIN;
+#pragma unroll
+  for (int i = 0; i < NOUT; ++i)
+#pragma unroll
+    for (int j = 0; j < NOUT; ++j) V[i][j] = md.par64(base + i * NOUT + j);
+}
+
 template <typename R, int NIN, int NOUT, typename S>
 PSM_DEV void model_value(const ModelView<R, S>& md, const R (&x)[NIN], R (&y)[NOUT]) {
   R J[NOUT][NIN];
@@ -207,6 +224,9 @@ PSM_DEV void linearize_extended(const ModelView<R, S>& md, const R (&m)[NIN], R 
 #pragma unroll
     for (int j = 0; j < NOUT; ++j) C[i][j] = md.qc(i * NOUT + j);
   }
+  // conditional moments (parsmooth/linearization/_extended.py:43-56): the
+  // conditional factor at the mean replaces the noise
+  if (md.kind == PSM_MODEL_CONDITIONAL) model_cond_factor<R, NIN, NOUT>(md, C);
 }
 
 // ----------------------------------------------------------------------------
@@ -295,11 +315,38 @@ PSM_DEV void linearize_sigma(const ModelView<R, S>& md, const SigmaView<R, S>& s
 #pragma unroll
       for (int i = 0; i < NOUT; ++i) Acc[i][NOUT] = sw * d[i];
       tria_inplace<R, NOUT, NOUT + 1>(Acc);
+      if (md.kind == PSM_MODEL_CONDITIONAL) {
+        // conditional moments (parsmooth/linearization/_sigma_points.py:25-50):
+        // the columns sqrt(wc_p) chol(x_p) join the factor
+        R V[NOUT][NOUT], Ac[NOUT][2 * NOUT];
+        model_cond_factor<R, NIN, NOUT>(mp, V);
+#pragma unroll
+        for (int i = 0; i < NOUT; ++i)
+#pragma unroll
+          for (int j = 0; j < NOUT; ++j) {
+            Ac[i][j] = Acc[i][j];
+            Ac[i][NOUT + j] = sw * V[i][j];
+          }
+        tria_inplace<R, NOUT, 2 * NOUT>(Ac);
+#pragma unroll
+        for (int i = 0; i < NOUT; ++i)
+#pragma unroll
+          for (int j = 0; j < NOUT; ++j) Acc[i][j] = Ac[i][j];
+      }
     } else {
 #pragma unroll
       for (int i = 0; i < NOUT; ++i)
 #pragma unroll
         for (int j = 0; j < NOUT; ++j) Phi[i][j] = fma(w * d[i], d[j], Phi[i][j]);
+      if (md.kind == PSM_MODEL_CONDITIONAL) {
+        // (parsmooth/linearization/_sigma_points.py:52-60): + sum_p wc_p V(x_p)
+        R V[NOUT][NOUT];
+        model_cond_factor<R, NIN, NOUT>(mp, V);
+#pragma unroll
+        for (int i = 0; i < NOUT; ++i)
+#pragma unroll
+          for (int j = 0; j < NOUT; ++j) Phi[i][j] = fma(w, V[i][j], Phi[i][j]);
+      }
     }
   }
   // F = (L^-T L^-1 Psi)^T

@@ -173,7 +173,7 @@ class Models:
                                  ("obs", self.obs_fn, self.obs if obs else None)):
             if struct is None or not isinstance(fn, TorchFunction):
                 continue
-            width = (xi.shape[0] * fn.nout) if xi is not None else fn.nout * (1 + fn.nin)
+            width = fn.row_width(xi.shape[0] if xi is not None else 0)
             buf = self._tables.get(name)
             if buf is None or buf.shape != (self.prob.n_states, width):
                 buf = torch.empty((self.prob.n_states, width), dtype=self.prob.dtype, device=self.prob.dev)

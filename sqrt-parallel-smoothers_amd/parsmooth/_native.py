@@ -26,6 +26,7 @@ MODEL_COS = 4
 MODEL_EXP = 5
 MODEL_ARCTAN = 6
 MODEL_TABULATED = 7
+MODEL_CONDITIONAL = 8
 
 LIB_NAME = "libparsmooth_hip.so"
 LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), LIB_NAME)

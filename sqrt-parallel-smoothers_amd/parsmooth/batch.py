@@ -15,13 +15,15 @@ import numpy as np
 import torch
 
 from . import _native as N
-from ._base import MVNSqrt, MVNStandard
+from ._base import ConditionalMomentsModel, MVNSqrt, MVNStandard
 from ._engine import Models, Problem, filter_pass, filter_smoother_pass, to_dev
 from .linearization._methods import resolve
 
 
 def _noise_dim(model) -> int:
     """Output dimension of a FunctionalModel: the size of its noise mean."""
+    if isinstance(model, ConditionalMomentsModel):
+        raise TypeError("pass nx (and ny) explicitly for conditional-moments models")
     q = model[1][0]
     return int(q.numel()) if isinstance(q, torch.Tensor) else int(np.prod(np.shape(q)))
 
@@ -39,12 +41,13 @@ class IteratedSmoother:
     """
 
     def __init__(self, lengths: Sequence[int], transition_model, observation_model,
-                 linearization_method: Callable, dtype=torch.float64, sqrt: bool = True, chunk: int = 0):
+                 linearization_method: Callable, dtype=torch.float64, sqrt: bool = True, chunk: int = 0,
+                 nx: int = None, ny: int = None):
         self.dev = N.device()
         self.dtype = dtype
         self.sqrt = sqrt
-        nx = _noise_dim(transition_model)
-        ny = _noise_dim(observation_model)
+        nx = int(nx) if nx is not None else _noise_dim(transition_model)
+        ny = int(ny) if ny is not None else _noise_dim(observation_model)
         self.prob = Problem(nx, ny, lengths, dtype, sqrt, self.dev, chunk)
         self.mods = Models(self.prob, transition_model, observation_model, resolve(linearization_method))
         self.nx, self.ny = nx, ny

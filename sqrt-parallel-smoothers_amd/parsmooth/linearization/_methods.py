@@ -11,7 +11,7 @@ import torch
 
 from .. import _native as N
 from .._base import ConditionalMomentsModel, FunctionalModel, MVNSqrt, MVNStandard, are_inputs_compatible
-from ..models import TorchFunction, as_device_function
+from ..models import ConditionalMoments, TorchFunction, as_device_function
 from . import _tables
 
 
@@ -77,14 +77,18 @@ def resolve(method) -> Method:
     return m
 
 
+def _conditional_function(model: ConditionalMomentsModel, nin: int, dtype, dev) -> ConditionalMoments:
+    probe = model.conditional_mean(torch.zeros(nin, dtype=dtype, device=dev))
+    nout = int(np.prod(np.shape(probe)))
+    return ConditionalMoments(model.conditional_mean, model.conditional_covariance_or_cholesky, nin, nout)
+
+
 def model_function(model: FunctionalModel, nin: int):
     """The device function of a FunctionalModel: a parsmooth.models function as
     is, any other callable wrapped in a TorchFunction (input dimension nin,
     output dimension that of the noise mean)."""
-    if isinstance(model, ConditionalMomentsModel):
-        raise NotImplementedError("ConditionalMomentsModel is not supported by the HIP path")
     if not isinstance(model, FunctionalModel):
-        raise TypeError("model must be a FunctionalModel")
+        raise TypeError("model must be a FunctionalModel or a ConditionalMomentsModel")
     f, q = model
     nout = int(np.prod(np.shape(q[0]))) if not isinstance(q[0], torch.Tensor) else int(q[0].numel())
     return as_device_function(f, nin, nout)
@@ -94,6 +98,13 @@ def model_struct(model: FunctionalModel, dtype, dev, sqrt: bool, nin: int = None
     """PsmModel for a FunctionalModel plus the tensors it points to (and the
     device function).  For a TorchFunction the value table is attached per call
     (``params`` is set by whoever tabulates)."""
+    if isinstance(model, ConditionalMomentsModel):
+        # no additive noise: zero mean and factor, the conditional factors come with the table
+        f = _conditional_function(model, int(nin), dtype, dev)
+        qm = torch.zeros(f.nout, dtype=dtype, device=dev)
+        qc = torch.zeros((f.nout, f.nout), dtype=dtype, device=dev)
+        s = N.PsmModel(f.kind, f.nin, f.nout, 0, 0, qm.data_ptr(), qc.data_ptr())
+        return s, (None, qm, qc, f)
     f = model_function(model, nin if nin is not None else getattr(model.function, "nin", 0))
     q = model[1]
     if sqrt != isinstance(q, MVNSqrt):
@@ -112,10 +123,8 @@ def model_struct(model: FunctionalModel, dtype, dev, sqrt: bool, nin: int = None
 
 
 def linearize(model, x, method: Method):
-    if isinstance(model, ConditionalMomentsModel):
-        raise NotImplementedError("ConditionalMomentsModel is not supported by the HIP path")
-    f, q = model
-    are_inputs_compatible(x, q)
+    if isinstance(model, FunctionalModel):
+        are_inputs_compatible(x, model[1])
     sqrt = isinstance(x, MVNSqrt)
     mean = x[0]
     dtype = mean.dtype if isinstance(mean, torch.Tensor) else torch.float64

@@ -195,6 +195,10 @@ class TorchFunction(DeviceFunction):
     def __call__(self, x):
         return self.fn(x)
 
+    def row_width(self, npts: int) -> int:
+        """Values per table row (npts = 0: extended linearization)."""
+        return npts * self.nout if npts else self.nout * (1 + self.nin)
+
     def _eval(self, x):
         y = self.fn(x)
         if not isinstance(y, torch.Tensor):
@@ -230,3 +234,45 @@ def as_device_function(f, nin: int, nout: int) -> DeviceFunction:
     if not callable(f):
         raise TypeError(f"model function must be callable, got {f!r}")
     return TorchFunction(f, nin, nout)
+
+
+class ConditionalMoments(TorchFunction):
+    """A ``ConditionalMomentsModel`` (E[y | x], and chol or covariance of
+    y | x) evaluated by torch: the table rows carry the conditional mean and,
+    after it, the conditional factor at every linearization point (the mean
+    for the extended linearization, each sigma point otherwise)."""
+
+    kind = N.MODEL_CONDITIONAL
+
+    def __init__(self, mean_fn, factor_fn, nin: int, nout: int):
+        super().__init__(mean_fn, nin, nout)
+        self.factor_fn = factor_fn
+
+    def row_width(self, npts: int) -> int:
+        v = self.nout * self.nout
+        return npts * (self.nout + v) if npts else self.nout * (1 + self.nin) + v
+
+    def _factor(self, x):
+        v = torch.func.vmap(lambda z: torch.as_tensor(self.factor_fn(z), dtype=x.dtype, device=x.device)
+                            .reshape(self.nout, self.nout))(x)
+        return v.reshape(x.shape[0], self.nout * self.nout)
+
+    def _mean(self, x):
+        return torch.func.vmap(lambda z: torch.as_tensor(self.fn(z), dtype=x.dtype, device=x.device)
+                               .reshape(self.nout))(x)
+
+    def tabulate(self, m: torch.Tensor, L: torch.Tensor, xi, out: torch.Tensor = None):
+        n = m.shape[0]
+        if xi is None:
+            f1 = lambda v: torch.as_tensor(self.fn(v), dtype=v.dtype, device=v.device).reshape(self.nout)
+            J = torch.func.vmap(torch.func.jacfwd(f1))(m)
+            rows = torch.cat([self._mean(m), J.reshape(n, self.nout * self.nin), self._factor(m)], dim=1)
+        else:
+            xi = xi.to(dtype=m.dtype, device=m.device)
+            chi = (m[:, None, :] + torch.einsum("nij,pj->npi", L, xi)).reshape(-1, self.nin)
+            rows = torch.cat([self._mean(chi), self._factor(chi)], dim=1).reshape(n, -1)
+        rows = rows.to(m.dtype)
+        if out is None:
+            return rows.contiguous()
+        out.copy_(rows)
+        return out

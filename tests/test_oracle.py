@@ -416,3 +416,108 @@ def test_sampler_sqrt_equals_std_for_the_same_normals():
     xs2 = O.sampling(eps, tq_std, O.MVNStandard(filt.mean, gram(filt.chol)), O.extended,
                      O.MVNStandard(smooth.mean, gram(smooth.chol)))
     np.testing.assert_allclose(xs2, xs, rtol=1e-9, atol=1e-9)
+
+
+# ---------------------------------------------------------------- conditional moments
+class _CondLinear:
+    """E[y | x] = a x + b m_q + c (tests/test_linearization.py:25-26)."""
+
+    def __init__(self, a, bq):
+        self.a, self.bq = a, bq
+
+    def __call__(self, x):
+        return np.asarray(x) @ self.a.T + self.bq
+
+    def jac(self, x):
+        return np.broadcast_to(self.a, np.asarray(x).shape[:-1] + self.a.shape).copy()
+
+
+class _CondConst:
+    def __init__(self, M):
+        self.M = M
+
+    def __call__(self, x):
+        return np.broadcast_to(self.M, np.asarray(x).shape[:-1] + self.M.shape).copy()
+
+
+@pytest.mark.parametrize("dim_x", [1, 3])
+@pytest.mark.parametrize("dim_q", [1, 2, 3])
+@pytest.mark.parametrize("method", list(METHODS))
+@pytest.mark.parametrize("sqrt", [True, False])
+def test_linear_conditional_is_exact(dim_x, dim_q, method, sqrt):
+    """tests/test_linearization.py:110-149."""
+    np.random.seed(dim_x * 7 + dim_q)
+    a, b, c = np.random.randn(dim_x, dim_x), np.random.randn(dim_x, dim_q), np.random.randn(dim_x)
+    m_x, m_q = np.random.randn(dim_x), np.random.randn(dim_q)
+    chol_x = np.tril(np.random.rand(dim_x, dim_x))
+    chol_q = np.tril(np.random.rand(dim_q, dim_q))
+    Bq = b @ chol_q
+    chol_f = np.concatenate([Bq, np.zeros((dim_x, dim_x - dim_q))], 1) if dim_x > dim_q else O.tria(Bq)
+    mean = _CondLinear(a, b @ m_q + c)
+    if sqrt:
+        model = O.ConditionalMomentsModel(mean, _CondConst(chol_f))
+        x = O.MVNSqrt(m_x, chol_x)
+    else:
+        model = O.ConditionalMomentsModel(mean, _CondConst(Bq @ Bq.T))
+        x = O.MVNStandard(m_x, chol_x @ chol_x.T)
+    F, Ql, rem = METHODS[method](model, x)
+    if sqrt:
+        Ql = Ql @ Ql.T
+    xp = np.random.randn(dim_x)
+    np.testing.assert_allclose(F, a, atol=1e-7)
+    np.testing.assert_allclose(F @ xp + rem, a @ xp + b @ m_q + c, atol=1e-7)
+    np.testing.assert_allclose(Ql, Bq @ Bq.T, atol=1e-7)
+
+
+class _PopMean:
+    """Population model: log(44.7) + x - exp(x) (tests/test_linearization.py:45-46)."""
+
+    def __call__(self, x):
+        return np.log(44.7) + np.asarray(x) - np.exp(x)
+
+    def jac(self, x):
+        x = np.asarray(x)
+        return (1.0 - np.exp(x))[..., :, None] * np.eye(x.shape[-1])
+
+
+class _PoissonMean:
+    """lam exp(x) (tests/test_linearization.py:57-58)."""
+
+    def __init__(self, lam):
+        self.lam = lam
+
+    def __call__(self, x):
+        return self.lam * np.exp(np.asarray(x))
+
+    def jac(self, x):
+        x = np.asarray(x)
+        return (self.lam * np.exp(x))[..., :, None] * np.eye(x.shape[-1])
+
+
+class _PoissonFactor:
+    def __init__(self, lam, sqrt):
+        self.lam, self.sqrt = lam, sqrt
+
+    def __call__(self, x):
+        v = self.lam * np.exp(np.asarray(x))
+        return (np.sqrt(v) if self.sqrt else v)[..., None]
+
+
+@pytest.mark.parametrize("seed", [0, 42])
+@pytest.mark.parametrize("method", list(METHODS))
+def test_conditional_sqrt_vs_std(seed, method):
+    """tests/test_linearization.py:180-233: the population model's transition
+    (constant noise 0.3^2) and Poisson observation (lam = 10)."""
+    np.random.seed(seed)
+    m_x = np.random.randn(1)
+    chol_x = np.random.rand(1, 1)
+    pairs = [(O.ConditionalMomentsModel(_PopMean(), _CondConst(np.array([[0.3]]))),
+              O.ConditionalMomentsModel(_PopMean(), _CondConst(np.array([[0.09]])))),
+             (O.ConditionalMomentsModel(_PoissonMean(10.0), _PoissonFactor(10.0, True)),
+              O.ConditionalMomentsModel(_PoissonMean(10.0), _PoissonFactor(10.0, False)))]
+    for sq_model, std_model in pairs:
+        Fs, Ls, bs = METHODS[method](sq_model, O.MVNSqrt(m_x, chol_x))
+        F, P, b = METHODS[method](std_model, O.MVNStandard(m_x, chol_x @ chol_x.T))
+        np.testing.assert_allclose(Fs, F, atol=1e-10)
+        np.testing.assert_allclose(bs, b, atol=1e-10)
+        np.testing.assert_allclose(Ls @ Ls.T, P, atol=1e-10)
