@@ -262,6 +262,13 @@ def fp32_cubature_errors(exps, n_iter):
     return res
 
 
+def _sched(tuned, runner):
+    """The launch schedule a runner uses: chunk, scan fan-in, team upper levels."""
+    k = int(runner.prob.struct.scan_k)
+    return {"chunk": runner.chunk_size(), "scan_k": (k & 0xFF) or 4, "team_upper_levels": bool(k & 0x100),
+            "autotuned": tuned is not None, "tuned_ms_per_pass": None if tuned is None else tuned[2]}
+
+
 def time_runner(runner, init, n_iter, steps, warmup, use_graph, kernel_timing=False):
     import torch
     from parsmooth import _native as N
@@ -330,6 +337,10 @@ def main():
     ap.add_argument("--grad-seqs", type=int, default=256)
     ap.add_argument("--grad-T", type=int, default=16384)
     ap.add_argument("--grad-iter", type=int, default=10)
+    ap.add_argument("--autotune", type=int, default=1,
+                    help="pick the launch schedule with psm_autotune before timing (1) or keep the default (0)")
+    ap.add_argument("--schedule", type=str, default="",
+                    help="fixed schedule 'chunk,scan_k,team' for the T=2^17 legs (implies --autotune 0)")
     ap.add_argument("--sweep", type=str, default="10,12,14,16",
                     help="comma-separated log2 T of the IEKS T-sweep ('' to skip)")
     ap.add_argument("--fp32-errors", type=str, default="14,16,18",
@@ -358,7 +369,17 @@ def main():
 
     # FP64 (headline), with live per-kernel HIP-event timing in a separate pass
     log(f"FP64 IEKS T={T} n_iter={n_iter}")
+    fixed = [int(v) for v in args.schedule.split(",")] if args.schedule else None
+    if fixed:
+        args.autotune = 0
+
+    def apply_fixed(r):
+        if fixed:
+            r.prob.set_schedule(fixed[0], fixed[1] | (0x100 if fixed[2] else 0))
+
     r64, init64 = build_runner(T, ys, torch.float64)
+    apply_fixed(r64)
+    sched64 = r64.autotune(init64) if args.autotune else None
     dt64, out64, _ = time_runner(r64, init64, n_iter, args.steps, args.warmup, bool(args.graph))
     dtk, _, rep = time_runner(r64, init64, n_iter, max(1, args.steps // 2), 0, False, kernel_timing=True)
     mean64 = out64[0].double().cpu().numpy()
@@ -366,6 +387,8 @@ def main():
     # FP32
     log("FP32 IEKS")
     r32, init32 = build_runner(T, ys, torch.float32)
+    apply_fixed(r32)
+    sched32 = r32.autotune(init32) if args.autotune else None
     r32.prob.fallback_counter.zero_()
     dt32, out32, _ = time_runner(r32, init32, n_iter, args.steps, args.warmup, bool(args.graph))
     fb32 = r32.fallback_count() / max(1, args.steps + args.warmup)
@@ -387,11 +410,13 @@ def main():
         from parsmooth.linearization import cubature
         log(f"batch leg: {args.batch_seqs} sequences")
         rb, ib, nsteps = build_batch_runner(args.batch_seqs, 256, 8192, 0, torch.float32, cubature)
+        schedb = rb.autotune(ib) if args.autotune else None
         dtb, _, repb = time_runner(rb, ib, 5, max(1, args.steps // 2), 1, bool(args.graph))
         _, _, repb = time_runner(rb, ib, 5, 1, 0, False, kernel_timing=True)
         batch = {"workload": f"{args.batch_seqs} bearings-only sequences, lengths U[256, 8192], FP32 sqrt "
                              "cubature IPLS, 5 iterations, one launch sequence per pass",
                  "steps": nsteps, "ms_per_step": dtb * 1e3, "value": nsteps * 5 / dtb,
+                 "schedule": _sched(schedb, rb),
                  "unit": "step-it/s", "kernel_ms": {k: v[0] / v[1] for k, v in repb.items()}}
         del rb, ib
         torch.cuda.empty_cache()
@@ -464,7 +489,9 @@ def main():
         "ms_per_step": dt64 * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "f64", "data": "synthetic (seeded bearings-only simulator)",
         "config": {"workload": "bearings-only coordinated turn, extended IEKS, n_iter=10, one sequence",
-                   "T": T, "nx": NX, "ny": NY, "n_seq": 1, "n_iter": n_iter, "graph": bool(args.graph)},
+                   "T": T, "nx": NX, "ny": NY, "n_seq": 1, "n_iter": n_iter, "graph": bool(args.graph),
+                   "schedule": _sched(sched64, r64)},
+        "fp32_schedule": _sched(sched32, r32),
         "fp32_value": T * n_iter / dt32, "fp32_ms_per_step": dt32 * 1e3, "fp32_over_fp64": dt64 / dt32,
         "fp32_rel_err_single_pass": rel32_pass, "fp32_rel_err_after_iterations": rel32_end,
         "fp32_fp64_fallbacks_per_step": fb32,
