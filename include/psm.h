@@ -97,6 +97,8 @@ typedef struct psm_sigma {
 
 #define PSM_SCAN_K_MASK 0xff
 #define PSM_SCAN_TEAM 0x100
+#define PSM_SCAN_TOP_SHIFT 16
+#define PSM_SCAN_TOP_MASK 0xff0000
 
 /* A ragged batch of sequences sharing dims, models and linearization. */
 typedef struct psm_problem {
@@ -115,9 +117,14 @@ typedef struct psm_problem {
    * disables the check.  fallback_count (device, may be NULL) is incremented
    * once per recomputation; the caller zeroes it. */
   float fallback_ratio;
-  /* bits 0-7: scan items folded per lane (0: PSM_SCAN_K or 4);
+  /* bits 0-7: scan items folded per lane (0: the PSM_SCAN_K environment
+   * variable, else 2 with team-cooperative upper levels and 4 without);
    * PSM_SCAN_TEAM: levels above the first run team-cooperative kernels
-   * (square-root form, float / double) */
+   * (square-root form, float / double); with bits 0-7 clear they do so by
+   * default, unless the PSM_SCAN_TEAM environment variable is 0;
+   * bits 16-23 (PSM_SCAN_TOP_MASK): a level whose items fit one workgroup
+   * with at most this many items per lane is the top level (values below the
+   * fan-in, 0 included, mean the fan-in) */
   int32_t scan_k;
   unsigned long long* fallback_count;
 } psm_problem_t;

@@ -70,7 +70,7 @@ inline size_t align_up(size_t x) { return (x + 255) & ~size_t(255); }
 // keep every CU busy on a single long sequence, larger when the batch is big.
 inline int default_chunk(int64_t M) {
   if (M >= (int64_t)1 << 22) return 16;
-  if (M >= (int64_t)1 << 18) return 8;
+  if (M >= (int64_t)1 << 17) return 8;
   return 4;
 }
 
@@ -94,7 +94,9 @@ inline Plan make_plan(const psm_problem_t* p, size_t rsize, int nlin, int nqf, i
   pl.off_selem = off;
   off = align_up(off + (size_t)nqs * pl.Mp * rsize);
   const int kreq = p->scan_k & PSM_SCAN_K_MASK;
-  const int kmax = kreq > 0 ? kreq : env_int("PSM_SCAN_K", 4);
+  const int kmax = kreq > 0 ? kreq : env_int("PSM_SCAN_K", wgu != wg ? 2 : 4);
+  const int ktop_req = (p->scan_k & PSM_SCAN_TOP_MASK) >> PSM_SCAN_TOP_SHIFT;
+  const int ktop = ktop_req > kmax ? ktop_req : kmax;
   int64_t n = pl.nchunks;
   int l = 0;
   while (true) {
@@ -102,7 +104,9 @@ inline Plan make_plan(const psm_problem_t* p, size_t rsize, int nlin, int nqf, i
     s.n = n;
     s.wg = l == 0 ? wg : wgu;
     const int wg_l = s.wg;
-    if (n <= (int64_t)wg_l * 16) {
+    // a level that fits one workgroup with at most ktop items per lane is the
+    // top; the others fold kmax items per lane
+    if (n <= (int64_t)wg_l * ktop) {
       s.K = (int)((n + wg_l - 1) / wg_l);
       if (s.K < 1) s.K = 1;
       s.nwg = 1;
@@ -186,16 +190,19 @@ struct Impl {
 
   using Acc = typename AccOf<R>::type;
   // Level 0 of a scan (the chunk aggregates: many items) runs one lane per
-  // combination with WG lanes per workgroup.  On request (PSM_SCAN_TEAM in
-  // scan_k, or the PSM_SCAN_TEAM environment variable), the square-root form
-  // in float / double runs the levels above (few items, a latency chain) with
-  // the team-cooperative kernels, kTeamVL teams per workgroup.
+  // combination with WG lanes per workgroup.  The square-root form in float /
+  // double runs the levels above (few items, a latency chain) with the
+  // team-cooperative kernels, kTeamVL teams per workgroup, when scan_k has the
+  // PSM_SCAN_TEAM bit, or by default (scan_k without a fan-in; the
+  // PSM_SCAN_TEAM environment variable set to 0 turns the default off).
   static constexpr bool TEAM = SQRT && (std::is_same<R, float>::value ||
                                         std::is_same<R, double>::value);
   static constexpr int WG = scan_wg<FA, R>() < scan_wg<SA, R>() ? scan_wg<FA, R>() : scan_wg<SA, R>();
 
   static bool team_requested(const psm_problem_t* p) {
-    return (p->scan_k & PSM_SCAN_TEAM) != 0 || env_int("PSM_SCAN_TEAM", 0) != 0;
+    if (p->scan_k & PSM_SCAN_TEAM) return true;
+    if (p->scan_k & PSM_SCAN_K_MASK) return false;  // explicit fan-in without the team bit
+    return env_int("PSM_SCAN_TEAM", 1) != 0;
   }
 
   static Plan plan(const psm_problem_t* p) {
@@ -331,7 +338,7 @@ struct Impl {
       bool launched = false;
       if constexpr (TEAM) {
         if (l > 0 && pl.team) {
-          hipLaunchKernelGGL((k_scan_down_team<Agg, R>), dim3((unsigned)s.nwg),
+          hipLaunchKernelGGL((k_scan_down_team<Agg, R>), dim3((unsigned)(s.nwg * s.K)),
                              dim3(TeamCfg<Agg, R>::WGT), 0, st, (const R*)(ws + s.off_out),
                              level_map(pl, l, 0), (const R*)(ws + upper_off),
                              level_map(pl, l + 1, 0), (R*)(ws + s.off_in), fb);
@@ -339,7 +346,8 @@ struct Impl {
         }
       }
       if (!launched)
-        hipLaunchKernelGGL((k_scan_down<Agg, R, SQRT, WG>), dim3((unsigned)s.nwg), dim3(WG), 0,
+        hipLaunchKernelGGL((k_scan_down<Agg, R, SQRT, WG>),
+                           dim3((unsigned)grid_for(s.nwg * s.K * s.wg, WG)), dim3(WG), 0,
                            st, (const R*)(ws + s.off_out), level_map(pl, l, 0),
                            (const R*)(ws + upper_off), level_map(pl, l + 1, 0),
                            (R*)(ws + s.off_in), fb);

@@ -318,33 +318,34 @@ __global__ __launch_bounds__(WG) void k_scan_up(const R* __restrict__ in, LevelM
     const int64_t item = lm.at(p);
     const V o{out_excl, lm.stride, item};
     view_copy(run, o);
-    combine_mem_fb<R, NX, SQRT>(o, V{inm, lm.stride, item}, run, fb);
+    // the running prefix after the lane's last item is its inclusive total,
+    // which the scan over the lanes already produced
+    if (j + 1 < K && p + 1 < n) combine_mem_fb<R, NX, SQRT>(o, V{inm, lm.stride, item}, run, fb);
   }
 }
 
 // dst[p] <- upper[w] (+) out_excl[p] (workgroup w > 0; w == 0 copies): the
 // complete exclusive prefixes, written out of place so the operands stay
 // intact for the FP64 fallback.  upper holds the complete exclusive prefixes of
-// the next level (um).
+// the next level (um).  The combinations are independent, so every thread
+// takes one storage position q of the level (consecutive threads, consecutive
+// positions) and recovers its scan position p and workgroup w from q.
 template <typename Agg, typename R, bool SQRT, int WG = scan_wg<Agg, R>()>
 __global__ __launch_bounds__(WG) void k_scan_down(const R* __restrict__ out_excl, LevelMap lm,
                                                   const R* __restrict__ upper, LevelMap um,
                                                   R* __restrict__ dst, Fallback fb) {
   using V = typename ViewOf<Agg>::type;
   constexpr int NX = Agg_nx<Agg>::value;
-  const int t = threadIdx.x;
-  const int K = lm.K;
-  const int64_t base = ((int64_t)blockIdx.x * WG + t) * K;
-  const V up{const_cast<R*>(upper), um.stride, um.at((int64_t)blockIdx.x)};
-  for (int j = 0; j < K; ++j) {
-    const int64_t p = base + j;
-    if (p >= lm.n) break;
-    const int64_t item = lm.at(p);
-    const V e{const_cast<R*>(out_excl), lm.stride, item};
-    const V o{dst, lm.stride, item};
-    if (blockIdx.x == 0) view_copy(e, o);
-    else combine_mem_fb<R, NX, SQRT>(up, e, o, fb);
-  }
+  const int64_t q = (int64_t)blockIdx.x * WG + threadIdx.x;
+  const int64_t span = (int64_t)lm.K * lm.WG;
+  const int64_t w = q / span;
+  const int64_t r = q - w * span;
+  const int64_t p = w * span + (r % lm.WG) * lm.K + r / lm.WG;
+  if (p >= lm.n) return;
+  const V e{const_cast<R*>(out_excl), lm.stride, q};
+  const V o{dst, lm.stride, q};
+  if (w == 0) view_copy(e, o);
+  else combine_mem_fb<R, NX, SQRT>(V{const_cast<R*>(upper), um.stride, um.at(w)}, e, o, fb);
 }
 
 // Team versions of the two scan kernels for the square-root form: kTeamVL
@@ -415,6 +416,7 @@ __global__ __launch_bounds__((TeamCfg<Agg, R>::WGT)) void k_scan_up_team(
     const int64_t item = lm.at(p);
     team_sync();
     team_copy(run, V{out_excl, lm.stride, item}, r);
+    if (j + 1 == K || p + 1 == n) break;  // the team's inclusive total is not needed
     team_combine<R, NX>(run, V{inm, lm.stride, item}, tmp, r, my, fb);
     const V x = run;
     run = tmp;
@@ -422,6 +424,8 @@ __global__ __launch_bounds__((TeamCfg<Agg, R>::WGT)) void k_scan_up_team(
   }
 }
 
+// One item per team: block b covers workgroup b / K of the level and item
+// b % K of each of its teams.
 template <typename Agg, typename R>
 __global__ __launch_bounds__((TeamCfg<Agg, R>::WGT)) void k_scan_down_team(
     const R* __restrict__ out_excl, LevelMap lm, const R* __restrict__ upper, LevelMap um,
@@ -436,17 +440,15 @@ __global__ __launch_bounds__((TeamCfg<Agg, R>::WGT)) void k_scan_down_team(
   const int r = (int)threadIdx.x % TL;
   void* my = scr + t * Cfg::SCR;
   const int K = lm.K;
-  const int64_t base = ((int64_t)blockIdx.x * VL + t) * K;
-  const V up{const_cast<R*>(upper), um.stride, um.at((int64_t)blockIdx.x)};
-  for (int j = 0; j < K; ++j) {
-    const int64_t p = base + j;
-    if (p >= lm.n) break;
-    const int64_t item = lm.at(p);
-    const V e{const_cast<R*>(out_excl), lm.stride, item};
-    const V o{dst, lm.stride, item};
-    if (blockIdx.x == 0) team_copy(e, o, r);
-    else team_combine<R, NX>(up, e, o, r, my, fb);
-  }
+  const int64_t w = (int64_t)blockIdx.x / K;
+  const int j = (int)((int64_t)blockIdx.x - w * K);
+  const int64_t p = (w * VL + t) * K + j;
+  if (p >= lm.n) return;  // whole teams leave together; team_combine syncs only within the team
+  const int64_t item = lm.at(p);
+  const V e{const_cast<R*>(out_excl), lm.stride, item};
+  const V o{dst, lm.stride, item};
+  if (w == 0) team_copy(e, o, r);
+  else team_combine<R, NX>(V{const_cast<R*>(upper), um.stride, um.at(w)}, e, o, r, my, fb);
 }
 
 // ============================================================================

@@ -21,6 +21,14 @@ def test_header_declares_what_binding_binds():
     assert _declared() == sorted(N.EXPORTS)
 
 
+def test_header_constants_match_binding():
+    txt = open(os.path.join(ROOT, "include", "psm.h")).read()
+    defs = {k: int(v, 0) for k, v in re.findall(r"#define (PSM_[A-Z0-9_]+) (0x[0-9a-fA-F]+|\d+)", txt)}
+    assert defs["PSM_SCAN_K_MASK"] == N.SCAN_K_MASK and defs["PSM_SCAN_TEAM"] == N.SCAN_TEAM
+    assert defs["PSM_SCAN_TOP_SHIFT"] == N.SCAN_TOP_SHIFT and defs["PSM_SCAN_TOP_MASK"] == N.SCAN_TOP_MASK
+    assert defs["PSM_F32_TANGENT"] == N.PSM_F32_TANGENT and defs["PSM_F64_TANGENT"] == N.PSM_F64_TANGENT
+
+
 def test_library_exports_every_declared_symbol():
     lib = N.load_library()
     for name in _declared():
@@ -87,6 +95,12 @@ def test_schedule_fields_change_the_plan():
             sizes[chunk, k] = lib.psm_workspace_bytes(ctypes.byref(p))
             assert sizes[chunk, k] > 100001 * 8 * (71 + 56)
     assert len(set(sizes.values())) > 1
+    # a larger top-level fold limit can only remove levels (less workspace)
+    p = N.PsmProblem(N.PSM_F64, N.FORM_SQRT, 5, 2, 1, 4, 4093, 1)  # 1024 chunk aggregates
+    p.scan_k = 2
+    w2 = lib.psm_workspace_bytes(ctypes.byref(p))
+    p.scan_k = 2 | (16 << N.SCAN_TOP_SHIFT)
+    assert lib.psm_workspace_bytes(ctypes.byref(p)) < w2
     # autotuner argument checks happen before any launch
     p = N.PsmProblem(N.PSM_F64, N.FORM_SQRT, 5, 2, 1, 0, 1025, 1)
     m = N.PsmModel(N.MODEL_LINEAR, 5, 5, 0, 1, 1, 1)

@@ -389,7 +389,8 @@ def test_every_schedule_matches_oracle(chunk):
     schedule the autotuner may pick, against the oracle per sequence."""
     lengths = [300, 17, 1, 2500, 64]
     to, oo = _bearings_setup(2, seed=0)[1]
-    for k in (2, 4, 8, 2 | N.SCAN_TEAM, 8 | N.SCAN_TEAM):
+    top16 = 16 << N.SCAN_TOP_SHIFT  # top level may fold up to 16 items per lane
+    for k in (2, 4, 8, 2 | N.SCAN_TEAM, 8 | N.SCAN_TEAM, 2 | N.SCAN_TEAM | top16, 4 | top16):
         r, obs, m0, nom = _schedule_runner(lengths, 3, chunk, k)
         assert r.chunk_size() == chunk
         out = r.prob.new_traj()
@@ -404,11 +405,14 @@ def test_every_schedule_matches_oracle(chunk):
 
 
 def test_deep_scan_schedules_agree():
-    """T = 300000: one-, two- and three-level scans (chunk 32/K 8, chunk 4/K 4,
-    chunk 2/K 2) give the same pass to FP64 rounding."""
+    """T = 300000: two- to five-level scans (chunk 32/K 8 ... chunk 2/K 2, per-lane
+    or team upper levels, top level folding up to K or 16 items per lane) give
+    the same pass to FP64 rounding."""
     T = 300000
     outs = []
-    for chunk, k in ((32, 8), (4, 4), (2, 2), (4, 4 | N.SCAN_TEAM), (2, 2 | N.SCAN_TEAM)):
+    top16 = 16 << N.SCAN_TOP_SHIFT
+    for chunk, k in ((32, 8), (4, 4), (2, 2), (4, 4 | N.SCAN_TEAM), (2, 2 | N.SCAN_TEAM),
+                     (8, 2 | N.SCAN_TEAM | top16), (2, 2 | top16)):
         r, _, _, nom = _schedule_runner([T], 4, chunk, k)
         out = r.prob.new_traj()
         r.pass_once((t64(nom[0]), t64(nom[1])), out)
