@@ -263,9 +263,12 @@ def fp32_cubature_errors(exps, n_iter):
 
 
 def _sched(tuned, runner):
-    """The launch schedule a runner uses: chunk, scan fan-in, team upper levels."""
+    """The launch schedule a runner uses: chunk, scan fan-in, team upper levels,
+    half-wave level 0 (scan_k 0: the library default, fan-in 2 with team levels)."""
     k = int(runner.prob.struct.scan_k)
-    return {"chunk": runner.chunk_size(), "scan_k": (k & 0xFF) or 4, "team_upper_levels": bool(k & 0x100),
+    team = bool(k & 0x100) or (k & 0x1FF) == 0
+    return {"chunk": runner.chunk_size(), "scan_k": (k & 0xFF) or (2 if team else 4), "team_upper_levels": team,
+            "half_wave_level0": bool(k & 0x200),
             "autotuned": tuned is not None, "tuned_ms_per_pass": None if tuned is None else tuned[2]}
 
 
@@ -340,7 +343,7 @@ def main():
     ap.add_argument("--autotune", type=int, default=1,
                     help="pick the launch schedule with psm_autotune before timing (1) or keep the default (0)")
     ap.add_argument("--schedule", type=str, default="",
-                    help="fixed schedule 'chunk,scan_k,team' for the T=2^17 legs (implies --autotune 0)")
+                    help="fixed schedule 'chunk,scan_k,team[,half]' for the T=2^17 legs (implies --autotune 0)")
     ap.add_argument("--sweep", type=str, default="10,12,14,16",
                     help="comma-separated log2 T of the IEKS T-sweep ('' to skip)")
     ap.add_argument("--fp32-errors", type=str, default="14,16,18",
@@ -375,7 +378,8 @@ def main():
 
     def apply_fixed(r):
         if fixed:
-            r.prob.set_schedule(fixed[0], fixed[1] | (0x100 if fixed[2] else 0))
+            r.prob.set_schedule(fixed[0], fixed[1] | (0x100 if fixed[2] else 0)
+                                | (0x200 if len(fixed) > 3 and fixed[3] else 0))
 
     r64, init64 = build_runner(T, ys, torch.float64)
     apply_fixed(r64)

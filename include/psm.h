@@ -97,6 +97,7 @@ typedef struct psm_sigma {
 
 #define PSM_SCAN_K_MASK 0xff
 #define PSM_SCAN_TEAM 0x100
+#define PSM_SCAN_HALF 0x200
 #define PSM_SCAN_TOP_SHIFT 16
 #define PSM_SCAN_TOP_MASK 0xff0000
 
@@ -122,6 +123,9 @@ typedef struct psm_problem {
    * PSM_SCAN_TEAM: levels above the first run team-cooperative kernels
    * (square-root form, float / double); with bits 0-7 clear they do so by
    * default, unless the PSM_SCAN_TEAM environment variable is 0;
+   * PSM_SCAN_HALF: level 0 of the scan runs 32-lane workgroups (square-root
+   * form, float / double): twice the workgroups, half the LDS each, one
+   * scan-over-lanes step less;
    * bits 16-23 (PSM_SCAN_TOP_MASK): a level whose items fit one workgroup
    * with at most this many items per lane is the top level (values below the
    * fan-in, 0 included, mean the fan-in) */
@@ -235,12 +239,14 @@ int psm_mean_sq_diff(int32_t dtype, int64_t n, const void* a, const void* b, dou
                      void* stream);
 
 /* Schedule autotuner (no reference counterpart: the reference's schedule is
- * XLA's).  Times psm_filter_smoother (eager, on `stream`) for every candidate
- * schedule -- chunk in {2, 4, 8, 16, 32} states per lane, scan fan-in
- * scan_k in {2, 4, 8}, with and without PSM_SCAN_TEAM (square-root form in
- * float / double) -- whose workspace fits workspace_bytes, and returns the
- * fastest in *best_chunk / *best_scan_k (and its time per pass in *best_ms,
- * may be NULL).  prob->chunk / prob->scan_k are ignored.  The outputs
+ * XLA's).  Times psm_filter_smoother (eager, on `stream`) for candidate
+ * schedules whose workspace fits workspace_bytes: first chunk in {2, 4, 8, 16,
+ * 32} states per lane x scan fan-in scan_k in {2, 4, 8}, with and without
+ * PSM_SCAN_TEAM and PSM_SCAN_HALF (square-root form in float / double); then
+ * the chunks at 5/8 .. 7/4 of the best chunk (those that are not powers of
+ * two, at most 64) with the three best scan_k values.  Returns the fastest in
+ * *best_chunk / *best_scan_k (and its time per pass in *best_ms, may be
+ * NULL).  prob->chunk / prob->scan_k are ignored.  The outputs
  * (out_m, out_L, filt_m, filt_L) are overwritten; FP32 fallback counts
  * accumulate.  Synchronizes the stream.  Results are cached per (dtype, form,
  * nx, ny, n_seq, n_states, linearization) behind a mutex: a shape tuned before

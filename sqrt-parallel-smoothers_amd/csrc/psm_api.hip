@@ -4,6 +4,7 @@
 // hipGraph cache of psm_filter_smoother.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstring>
 #include <map>
 #include <mutex>
@@ -485,47 +486,88 @@ int psm_autotune(const psm_problem_t* p, const psm_model_t* trans, const psm_mod
     set_error("psm_autotune: hipEventCreate failed");
     return PSM_E_HIP;
   }
-  TuneResult best{0, 0, 0.0f};
+  // time one candidate: a warm-up pass, then kReps passes between events
+  constexpr int kReps = 2;
+  auto time_one = [&](int32_t c, int32_t k, float* ms) -> int {
+    psm_problem_t q = *p;
+    q.chunk = c;
+    q.scan_k = k;
+    *ms = -1.0f;
+    if (t->workspace_bytes(&q) > wsb) return PSM_OK;  // skipped: does not fit
+    int r = psm_filter_smoother(&q, trans, obs, sigma, ys, m0, L0, nom_m, nom_L, filt_m, filt_L,
+                                out_m, out_L, ws, wsb, 0, stream);
+    if (r) return r;
+    (void)hipEventRecord(e0, st);
+    for (int i = 0; i < kReps && r == 0; ++i)
+      r = psm_filter_smoother(&q, trans, obs, sigma, ys, m0, L0, nom_m, nom_L, filt_m, filt_L,
+                              out_m, out_L, ws, wsb, 0, stream);
+    (void)hipEventRecord(e1, st);
+    if (r) return r;
+    if (hipEventSynchronize(e1) != hipSuccess) {
+      set_error("psm_autotune: hipEventSynchronize failed");
+      return PSM_E_HIP;
+    }
+    float v = 0.0f;
+    (void)hipEventElapsedTime(&v, e0, e1);
+    *ms = v / kReps;
+    return PSM_OK;
+  };
+  // stage 1: power-of-two chunks x fan-in {2, 4, 8} x {per-lane, team upper
+  // levels, half-wave level 0, both}; stage 2: chunks between the stage-1
+  // sizes (5/8 .. 7/4 of the best one) with the three best scan modes.  Chunk
+  // sizes that are not powers of two avoid a nearly empty last workgroup when
+  // the state count is just above a power of two (T + 1 states).
   static const int32_t chunks[] = {2, 4, 8, 16, 32};
   static const int32_t ks[] = {2, 4, 8};
   const bool team_ok = p->form == PSM_FORM_SQRT && (p->dtype == PSM_F32 || p->dtype == PSM_F64);
-  constexpr int kReps = 2;
+  std::vector<TuneResult> res;
   for (int32_t c : chunks) {
-    for (int32_t kk = 0; kk < 6; ++kk) {
-      const int32_t k = ks[kk % 3] | (kk >= 3 ? PSM_SCAN_TEAM : 0);
-      if (kk >= 3 && !team_ok) continue;
-      psm_problem_t q = *p;
-      q.chunk = c;
-      q.scan_k = k;
-      if (t->workspace_bytes(&q) > wsb) continue;
-      rc = psm_filter_smoother(&q, trans, obs, sigma, ys, m0, L0, nom_m, nom_L, filt_m, filt_L,
-                               out_m, out_L, ws, wsb, 0, stream);
-      if (rc) break;
-      (void)hipEventRecord(e0, st);
-      for (int r = 0; r < kReps && rc == 0; ++r)
-        rc = psm_filter_smoother(&q, trans, obs, sigma, ys, m0, L0, nom_m, nom_L, filt_m, filt_L,
-                                 out_m, out_L, ws, wsb, 0, stream);
-      (void)hipEventRecord(e1, st);
-      if (rc) break;
-      if (hipEventSynchronize(e1) != hipSuccess) {
-        set_error("psm_autotune: hipEventSynchronize failed");
-        rc = PSM_E_HIP;
-        break;
-      }
-      float ms = 0.0f;
-      (void)hipEventElapsedTime(&ms, e0, e1);
-      ms /= kReps;
-      if (best.chunk == 0 || ms < best.ms) best = TuneResult{c, k, ms};
+    for (int32_t kk = 0; kk < 12 && rc == 0; ++kk) {
+      // kk / 3: 0 per-lane, 1 team upper levels, 2 half-wave level 0, 3 both
+      const int32_t mode = kk / 3;
+      const int32_t k = ks[kk % 3] | ((mode & 1) ? PSM_SCAN_TEAM : 0) | ((mode & 2) ? PSM_SCAN_HALF : 0);
+      if (mode > 0 && !team_ok) continue;
+      float ms;
+      rc = time_one(c, k, &ms);
+      if (rc == 0 && ms >= 0.0f) res.push_back(TuneResult{c, k, ms});
     }
     if (rc) break;
+  }
+  if (rc == 0 && !res.empty()) {
+    std::sort(res.begin(), res.end(),
+              [](const TuneResult& a, const TuneResult& b) { return a.ms < b.ms; });
+    std::vector<int32_t> modes;
+    for (const TuneResult& r : res) {
+      if (std::find(modes.begin(), modes.end(), r.scan_k) == modes.end()) modes.push_back(r.scan_k);
+      if (modes.size() == 3) break;
+    }
+    const int32_t c0 = res.front().chunk;
+    std::vector<int32_t> fine;
+    for (int32_t num : {5, 6, 7, 10, 12, 14}) {  // eighths of c0
+      const int32_t c = (c0 * num + 4) / 8;
+      if (c < 2 || c > 64 || (c & (c - 1)) == 0) continue;
+      if (std::find(fine.begin(), fine.end(), c) == fine.end()) fine.push_back(c);
+    }
+    for (int32_t c : fine) {
+      for (int32_t k : modes) {
+        float ms;
+        rc = time_one(c, k, &ms);
+        if (rc) break;
+        if (ms >= 0.0f) res.push_back(TuneResult{c, k, ms});
+      }
+      if (rc) break;
+    }
   }
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   if (rc) return rc;
-  if (best.chunk == 0) {
+  if (res.empty()) {
     set_error("psm_autotune: no candidate schedule fits the workspace");
     return PSM_E_WORKSPACE;
   }
+  TuneResult best = res.front();
+  for (const TuneResult& r : res)
+    if (r.ms < best.ms) best = r;
   g_tuned[key] = best;
   *best_chunk = best.chunk;
   *best_scan_k = best.scan_k;

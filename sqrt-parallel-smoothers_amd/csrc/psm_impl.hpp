@@ -68,16 +68,21 @@ inline size_t align_up(size_t x) { return (x + 255) & ~size_t(255); }
 
 // Default chunk (states per lane in the reduce / apply passes): small enough to
 // keep every CU busy on a single long sequence, larger when the batch is big.
+// Default states per chunk: 4, or larger so that the chunk aggregates fit one
+// round of 64-lane, fan-in-2 level-0 workgroups on 256 CUs (32768 items; one
+// aggregate more would run a nearly empty workgroup in a second round); 16 in
+// the throughput regime.
 inline int default_chunk(int64_t M) {
   if (M >= (int64_t)1 << 22) return 16;
-  if (M >= (int64_t)1 << 17) return 8;
-  return 4;
+  const int64_t c = (M + 32767) / 32768;
+  return c < 4 ? 4 : (int)c;
 }
 
 // wg: lanes per workgroup of level 0 (one lane per combination); wgu: lanes or
 // teams per workgroup of the levels above.
 inline Plan make_plan(const psm_problem_t* p, size_t rsize, int nlin, int nqf, int nqs,
-                      size_t accsize = sizeof(double), int wg = kScanWG, int wgu = 0) {
+                      size_t accsize = sizeof(double), int wg = kScanWG, int wgu = 0,
+                      int kdef = 4) {
   if (wgu <= 0) wgu = wg;
   Plan pl;
   pl.M = p->n_states;
@@ -94,7 +99,7 @@ inline Plan make_plan(const psm_problem_t* p, size_t rsize, int nlin, int nqf, i
   pl.off_selem = off;
   off = align_up(off + (size_t)nqs * pl.Mp * rsize);
   const int kreq = p->scan_k & PSM_SCAN_K_MASK;
-  const int kmax = kreq > 0 ? kreq : env_int("PSM_SCAN_K", wgu != wg ? 2 : 4);
+  const int kmax = kreq > 0 ? kreq : env_int("PSM_SCAN_K", kdef);
   const int ktop_req = (p->scan_k & PSM_SCAN_TOP_MASK) >> PSM_SCAN_TOP_SHIFT;
   const int ktop = ktop_req > kmax ? ktop_req : kmax;
   int64_t n = pl.nchunks;
@@ -207,7 +212,9 @@ struct Impl {
 
   static Plan plan(const psm_problem_t* p) {
     const bool team = TEAM && team_requested(p);
-    Plan pl = make_plan(p, sizeof(R), NLIN, FA::NQ, SA::NQ, sizeof(Acc), WG, team ? kTeamVL : WG);
+    const bool half = TEAM && (p->scan_k & PSM_SCAN_HALF) != 0;
+    Plan pl = make_plan(p, sizeof(R), NLIN, FA::NQ, SA::NQ, sizeof(Acc), half ? WG / 2 : WG,
+                        team ? kTeamVL : WG, team ? 2 : 4);
     pl.team = team;
     return pl;
   }
@@ -320,6 +327,13 @@ struct Impl {
           hipLaunchKernelGGL((k_scan_up_team<Agg, R>), dim3((unsigned)s.nwg),
                              dim3(TeamCfg<Agg, R>::WGT), 0, st, (const R*)in, level_map(pl, l, 0),
                              out, tot, tm, fb);
+          launched = true;
+        }
+      }
+      if constexpr (TEAM) {
+        if (!launched && s.wg == WG / 2) {  // half-wave level 0
+          hipLaunchKernelGGL((k_scan_up<Agg, R, SQRT, WG / 2>), dim3((unsigned)s.nwg), dim3(WG / 2),
+                             0, st, (const R*)in, level_map(pl, l, 0), out, tot, tm, fb);
           launched = true;
         }
       }

@@ -41,11 +41,11 @@
 namespace psm {
 
 constexpr int kScanWG = 64;     // one wave per scan workgroup
-// Scan workgroup size for an aggregate type: one wave, or half a wave when two
+// Scan workgroup size for an aggregate type: one wave, or half a wave when three
 // LDS buffers of 64 aggregates would not fit the 160 KB of LDS (tangent types).
 template <typename Agg, typename R>
 constexpr int scan_wg() {
-  return (2 * Agg::NQ * 64 * (int)sizeof(R) <= 150 * 1024) ? 64 : 32;
+  return (3 * Agg::NQ * 64 * (int)sizeof(R) <= 150 * 1024) ? 64 : 32;
 }
 constexpr int kLaneWG = 64;     // workgroup size of the lane-per-item kernels
 
@@ -281,20 +281,25 @@ __global__ __launch_bounds__(WG) void k_scan_up(const R* __restrict__ in, LevelM
   using V = typename ViewOf<Agg>::type;
   constexpr int NX = Agg_nx<Agg>::value;
   constexpr int NQ = Agg::NQ;
-  __shared__ R lds[2 * NQ * WG];
+  // buffers 0 and 1: running / scanned aggregates; buffer 2: the lane's next
+  // input item, copied from HBM in one burst so that the combination reads
+  // only LDS
+  __shared__ R lds[3 * NQ * WG];
   const int t = threadIdx.x;
   const int K = lm.K;
   const int64_t n = lm.n;
   auto slot = [&](int buf, int lane) { return V{lds + buf * NQ * WG, (int64_t)WG, (int64_t)lane}; };
   R* inm = const_cast<R*>(in);
+  const V stage = slot(2, t);
   const int64_t base = ((int64_t)blockIdx.x * WG + t) * K;
   int cur = 0;
-  view_identity(slot(0, t));
-  for (int j = 0; j < K; ++j) {
+  if (base < n) view_copy(V{inm, lm.stride, lm.at(base)}, slot(0, t));
+  else view_identity(slot(0, t));
+  for (int j = 1; j < K; ++j) {
     const int64_t p = base + j;
     if (p >= n) break;
-    const V e{inm, lm.stride, lm.at(p)};
-    combine_mem_fb<R, NX, SQRT>(slot(cur, t), e, slot(1 - cur, t), fb);
+    view_copy(V{inm, lm.stride, lm.at(p)}, stage);
+    combine_mem_fb<R, NX, SQRT>(slot(cur, t), stage, slot(1 - cur, t), fb);
     cur = 1 - cur;
   }
   if (cur == 1) view_copy(slot(1, t), slot(0, t));
@@ -308,19 +313,27 @@ __global__ __launch_bounds__(WG) void k_scan_up(const R* __restrict__ in, LevelM
   __syncthreads();
   if (t == WG - 1 && totals != nullptr)
     view_copy(slot(cur, t), V{totals, tm.stride, tm.at((int64_t)blockIdx.x)});
-  // running exclusive prefix in this lane's slot of the other buffer
-  const V run = slot(1 - cur, t);
+  // running exclusive prefix in this lane's slot of the other buffer; once
+  // every lane has read its predecessor's inclusive prefix, slot(cur, t) is
+  // free for the next running prefix
+  V run = slot(1 - cur, t);
   if (t == 0) view_identity(run);
   else view_copy(slot(cur, t - 1), run);
+  __syncthreads();
+  V tmp = slot(cur, t);
   for (int j = 0; j < K; ++j) {
     const int64_t p = base + j;
     if (p >= n) break;
     const int64_t item = lm.at(p);
-    const V o{out_excl, lm.stride, item};
-    view_copy(run, o);
+    view_copy(run, V{out_excl, lm.stride, item});
     // the running prefix after the lane's last item is its inclusive total,
     // which the scan over the lanes already produced
-    if (j + 1 < K && p + 1 < n) combine_mem_fb<R, NX, SQRT>(o, V{inm, lm.stride, item}, run, fb);
+    if (j + 1 == K || p + 1 == n) break;
+    view_copy(V{inm, lm.stride, item}, stage);
+    combine_mem_fb<R, NX, SQRT>(run, stage, tmp, fb);
+    const V x = run;
+    run = tmp;
+    tmp = x;
   }
 }
 

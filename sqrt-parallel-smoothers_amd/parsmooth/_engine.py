@@ -81,9 +81,11 @@ class Problem:
         self._ws = None
 
     # candidate schedules of the library's autotuner: chunk, and scan fan-in
-    # with or without the team-cooperative upper scan levels (N.SCAN_TEAM)
+    # with or without the team-cooperative upper scan levels (N.SCAN_TEAM) and
+    # the half-wave level 0 (N.SCAN_HALF)
     TUNE_CHUNKS = (2, 4, 8, 16, 32)
-    TUNE_SCAN_K = (2, 4, 8, 2 | 0x100, 4 | 0x100, 8 | 0x100)
+    TUNE_SCAN_K = tuple(k | f for f in (0, N.SCAN_TEAM, N.SCAN_HALF, N.SCAN_TEAM | N.SCAN_HALF)
+                        for k in (2, 4, 8))
 
     def set_schedule(self, chunk: int, scan_k: int):
         """Fix the launch schedule (0 = library default) and resize the workspace."""
@@ -101,7 +103,7 @@ class Problem:
         nominal and keep the fastest; returns (chunk, scan_k, ms per pass)."""
         saved = (self.struct.chunk, self.struct.scan_k)
         need = 0
-        for c in self.TUNE_CHUNKS:
+        for c in range(2, 65):  # the refinement stage may pick any chunk up to 64
             for k in self.TUNE_SCAN_K:
                 self.struct.chunk, self.struct.scan_k = c, k
                 need = max(need, int(N.lib().psm_workspace_bytes(ctypes.byref(self.struct))))

@@ -383,23 +383,36 @@ def _schedule_runner(lengths, seed, chunk, scan_k, dtype=torch.float64):
     return r, obs, m0, nom
 
 
-@pytest.mark.parametrize("chunk", [2, 4, 8, 16, 32])
-def test_every_schedule_matches_oracle(chunk):
-    """One filter-smoother pass over a ragged batch for each (chunk, scan fan-in)
-    schedule the autotuner may pick, against the oracle per sequence."""
-    lengths = [300, 17, 1, 2500, 64]
+_SCHED_LENGTHS = [300, 17, 1, 2500, 64]
+
+
+@functools.lru_cache(maxsize=1)
+def _schedule_refs():
+    """Oracle filter-smoother of the schedule-test batch (seed 3), per sequence."""
+    r, obs, m0, nom = _schedule_runner(_SCHED_LENGTHS, 3, 4, 0)
     to, oo = _bearings_setup(2, seed=0)[1]
+    o = r.state_offsets
+    return [O.filter_smoother(np.asarray(y, np.float64), O.MVNSqrt(m0[s], np.eye(5)), to, oo, O.extended,
+                              O.MVNSqrt(nom[0][o[s]:o[s + 1]], nom[1][o[s]:o[s + 1]]), True)
+            for s, y in enumerate(obs)]
+
+
+@pytest.mark.parametrize("chunk", [2, 3, 4, 5, 7, 8, 12, 16, 32])
+def test_every_schedule_matches_oracle(chunk):
+    """One filter-smoother pass over a ragged batch for each (chunk, scan fan-in,
+    scan mode) schedule the autotuner may pick, against the oracle per sequence."""
+    lengths = _SCHED_LENGTHS
+    refs = _schedule_refs()
     top16 = 16 << N.SCAN_TOP_SHIFT  # top level may fold up to 16 items per lane
-    for k in (2, 4, 8, 2 | N.SCAN_TEAM, 8 | N.SCAN_TEAM, 2 | N.SCAN_TEAM | top16, 4 | top16):
+    half = N.SCAN_HALF
+    for k in (2, 4, 8, 2 | N.SCAN_TEAM, 8 | N.SCAN_TEAM, 2 | N.SCAN_TEAM | top16, 4 | top16,
+              2 | half, 4 | N.SCAN_TEAM | half):
         r, obs, m0, nom = _schedule_runner(lengths, 3, chunk, k)
         assert r.chunk_size() == chunk
         out = r.prob.new_traj()
         r.pass_once((t64(nom[0]), t64(nom[1])), out)
         got_m, got_L = r.split(out[0]), r.split(out[1])
-        o = r.state_offsets
-        for s, y in enumerate(obs):
-            ref = O.filter_smoother(np.asarray(y, np.float64), O.MVNSqrt(m0[s], np.eye(5)), to, oo, O.extended,
-                                    O.MVNSqrt(nom[0][o[s]:o[s + 1]], nom[1][o[s]:o[s + 1]]), True)
+        for s, ref in enumerate(refs):
             close(npy(got_m[s]), ref.mean, what=f"chunk {chunk} K {k} seq {s} mean")
             close(gram(npy(got_L[s])), gram(ref.chol), what=f"chunk {chunk} K {k} seq {s} cov")
 
@@ -412,7 +425,7 @@ def test_deep_scan_schedules_agree():
     outs = []
     top16 = 16 << N.SCAN_TOP_SHIFT
     for chunk, k in ((32, 8), (4, 4), (2, 2), (4, 4 | N.SCAN_TEAM), (2, 2 | N.SCAN_TEAM),
-                     (8, 2 | N.SCAN_TEAM | top16), (2, 2 | top16)):
+                     (8, 2 | N.SCAN_TEAM | top16), (2, 2 | top16), (4, 2 | N.SCAN_TEAM | N.SCAN_HALF)):
         r, _, _, nom = _schedule_runner([T], 4, chunk, k)
         out = r.prob.new_traj()
         r.pass_once((t64(nom[0]), t64(nom[1])), out)
@@ -429,7 +442,7 @@ def test_autotune_result_matches_default_schedule():
     r, _, _, nom = _schedule_runner(lengths, 5, 0, 0)
     ref = [npy(x) for x in r.run(3, (nom[0], nom[1]), use_graph=False)]
     chunk, k, ms = r.autotune((nom[0], nom[1]))
-    assert chunk in r.prob.TUNE_CHUNKS and k in r.prob.TUNE_SCAN_K and ms > 0
+    assert 2 <= chunk <= 64 and k in r.prob.TUNE_SCAN_K and ms > 0
     assert r.chunk_size() == chunk
     got = [npy(x) for x in r.run(3, (nom[0], nom[1]), use_graph=True)]
     close(got[0], ref[0], tol=1e-8, what="autotuned mean")
@@ -469,6 +482,40 @@ def test_fp32_bearings_vs_fp64_oracle():
                                        criterion=lambda i, *_: i < 5)
     assert got.mean.dtype == torch.float32
     close(npy(got.mean), ref.mean, tol=1e-4, what="fp32 mean")
+
+
+@pytest.mark.parametrize("k", [2, 2 | N.SCAN_TEAM, 2 | N.SCAN_TEAM | N.SCAN_HALF, 4 | N.SCAN_HALF])
+def test_fp32_schedules_match_oracle(k):
+    """FP32 filter-smoother pass over a ragged batch with per-lane, team-cooperative
+    and half-wave scan schedules, against the FP64 oracle run on the same
+    FP32-rounded inputs (FP32 tolerances)."""
+    from parsmooth.batch import IteratedSmoother
+    from parsmooth.data import bearings_data_batch
+    lengths = [2000, 33, 1, 700]
+    rng = np.random.RandomState(8)
+    r32 = lambda a: np.asarray(a, np.float32).astype(np.float64)
+    m0 = r32(np.array([0.1, 0.2, 1.0, 0.0, 0.0]) + 0.3 * rng.randn(len(lengths), 5))
+    _, obs = bearings_data_batch(lengths, m0[:, :4], 0.01, 0.5, (-1.5, 0.5), (1.0, 1.0), seed=8)
+    obs = [r32(y) for y in obs]
+    _, (to, oo), (td64, od64) = _bearings_setup(2, seed=0)
+    t32 = lambda a: torch.as_tensor(np.asarray(a), dtype=torch.float32, device=DEV)
+    td = FunctionalModel(td64.function, MVNSqrt(t32(npy(td64.mvn.mean)), t32(npy(td64.mvn.chol))))
+    od = FunctionalModel(od64.function, MVNSqrt(t32(npy(od64.mvn.mean)), t32(npy(od64.mvn.chol))))
+    r = IteratedSmoother(lengths, td, od, extended, dtype=torch.float32)
+    r.prob.set_schedule(4, k)
+    r.set_data(t32(np.concatenate(obs)), t32(m0), t32(np.repeat(np.eye(5)[None], len(lengths), 0)))
+    M = r.prob.n_states
+    nom = (r32(np.repeat(m0, np.asarray(lengths) + 1, 0) + 0.05 * rng.randn(M, 5)),
+           r32(np.repeat(np.eye(5)[None] * 0.3, M, 0)))
+    out = r.prob.new_traj()
+    r.pass_once((t32(nom[0]), t32(nom[1])), out)
+    got_m, got_L = r.split(out[0]), r.split(out[1])
+    o = r.state_offsets
+    for s, y in enumerate(obs):
+        ref = O.filter_smoother(y, O.MVNSqrt(m0[s], np.eye(5)), to, oo, O.extended,
+                                O.MVNSqrt(nom[0][o[s]:o[s + 1]], nom[1][o[s]:o[s + 1]]), True)
+        close(npy(got_m[s]), ref.mean, tol=TOL32, what=f"fp32 K {k:#x} seq {s} mean")
+        close(gram(npy(got_L[s])), gram(ref.chol), tol=TOL32_COV, what=f"fp32 K {k:#x} seq {s} cov")
 
 
 def test_fp32_fallback_recomputes_in_fp64():
