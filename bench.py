@@ -43,25 +43,32 @@ PEAK_FP32 = 157.3e12
 PEAK_HBM = 8.0e12
 
 
-# rocprofv3 names of the kernels timed under a launch label (FP64, nx=5, ny=2)
+# rocprofv3 names of the kernels timed under a launch label (FP64, nx=5, ny=2);
+# {wg}: lanes of the level-0 scan workgroup (32 with the half-wave option)
 PMC_NAMES = {
-    "k_fscan_up_l0": "psm::k_scan_up<psm::FAgg<double, 5>, double, true, 64>",
-    "k_sscan_up_l0": "psm::k_scan_up<psm::SAgg<double, 5>, double, true, 64>",
+    "k_fscan_up_l0": "psm::k_scan_up<psm::FAgg<double, 5>, double, true, {wg}>",
+    "k_sscan_up_l0": "psm::k_scan_up<psm::SAgg<double, 5>, double, true, {wg}>",
     "k_filter_reduce": "psm::k_filter_reduce<double, 5, 2, true>",
     "k_filter_apply": "psm::k_filter_apply<double, 5, 2, true>",
     "k_smoother_reduce": "psm::k_smoother_reduce<double, 5, 2, true>",
     "k_smoother_apply": "psm::k_smoother_apply<double, 5, true>",
 }
+SCHED_KEYS = ("chunk", "scan_k", "team_upper_levels", "half_wave_level0")
 
 
-def measured_traffic(label):
+def measured_traffic(label, sched):
     """HBM bytes per (largest) dispatch of the kernel, from the FETCH_SIZE /
-    WRITE_SIZE passes of the same command summarised in profiles/ (scaled by a
-    copy kernel of known size); None when no such summary is present."""
+    WRITE_SIZE passes summarised in profiles/ (scaled by a copy kernel of known
+    size); None when no such summary is present or when it was taken with
+    another launch schedule than `sched`."""
     path = os.path.join(ROOT, "profiles", "pmc_traffic_bearings_ieks_T131072.json")
+    name = PMC_NAMES.get(label, "").format(wg=32 if sched.get("half_wave_level0") else 64)
     try:
         with open(path) as fh:
-            k = json.load(fh)["kernels"].get(PMC_NAMES.get(label, ""))
+            d = json.load(fh)
+        if any(d["schedule"][key] != sched[key] for key in SCHED_KEYS):
+            return None
+        k = d["kernels"].get(name)
     except (OSError, ValueError, KeyError):
         return None
     return None if k is None else k["hbm_bytes_max_scaled"]
@@ -478,7 +485,7 @@ def main():
     kflops = fl * units
     achieved = kflops / per_launch_s / 1e12
     roof = {"bound": "mfma", "achieved": achieved, "peak": PEAK_FP64 / 1e12, "unit": "TFLOP/s",
-            "frac": achieved / (PEAK_FP64 / 1e12), "traffic": measured_traffic(dom),
+            "frac": achieved / (PEAK_FP64 / 1e12), "traffic": measured_traffic(dom, _sched(sched64, r64)),
             "kernel": dom, "kernel_avg_us": per_launch_s * 1e6,
             "note": "FP64 VALU work (no MFMA on <=10x10 blocks); peak = MI355X FP64 vector rate"}
     cpu = None
