@@ -295,9 +295,10 @@ def test_iterated_bearings_vs_oracle(method):
     assert abs(float(ell_d) - ell_o) <= 1e-8 * abs(ell_o)
 
 
-@pytest.mark.parametrize("method", ["extended", "cubature"])
+@pytest.mark.parametrize("method", ["extended", "cubature", "gauss_hermite", "unscented"])
 def test_single_pass_bearings_1e9(method):
-    """One filter_smoother pass at an identical nominal: 1e-9 relative."""
+    """One filter_smoother pass at an identical nominal: 1e-9 relative (Gauss-Hermite:
+    order 3, 243 points; unscented: alpha 1, beta 0, kappa 3 + n, 11 points)."""
     T = 2000
     ys, (to, oo), (td, od) = _bearings_setup(T, seed=3)
     rng = np.random.RandomState(0)
@@ -309,6 +310,66 @@ def test_single_pass_bearings_1e9(method):
                                     MVNSqrt(t64(nom_m), t64(nom_c)))
     close(npy(got.mean), ref.mean, what="mean")
     close(gram(npy(got.chol)), gram(ref.chol), what="cov")
+
+
+@pytest.mark.parametrize("method", ["extended", "cubature", "gauss_hermite", "unscented"])
+def test_single_pass_bearings_standard_form(method):
+    """Covariance form (MVNStandard in and out) on the nonlinear bearings model: one
+    filter_smoother pass at an identical nominal against the oracle, 1e-9
+    relative, plus the filter log-likelihood."""
+    T = 500
+    ys, _, _ = _bearings_setup(T, seed=4)
+    Q, R, obs_f, trans_f = M.make_parameters(0.01, 0.1, 0.5, 0.01, (-1.5, 0.5), (1.0, 1.0))
+    to = O.FunctionalModel(O.CoordinatedTurn(0.01), O.MVNStandard(np.zeros(5), Q))
+    oo = O.FunctionalModel(O.Bearings((-1.5, 0.5), (1.0, 1.0)), O.MVNStandard(np.zeros(2), R))
+    td = FunctionalModel(trans_f, MVNStandard(t64(np.zeros(5)), t64(Q)))
+    od = FunctionalModel(obs_f, MVNStandard(t64(np.zeros(2)), t64(R)))
+    rng = np.random.RandomState(1)
+    nom_m = np.cumsum(0.01 * rng.randn(T + 1, 5), 0) + np.array([0.1, 0.2, 1.0, 0.0, 0.5])
+    nom_P = np.repeat(np.eye(5)[None] * 0.1, T + 1, 0)
+    m0 = np.array([0.1, 0.2, 1.0, 0.0, 0.0])
+    nom_o = O.MVNStandard(nom_m, nom_P)
+    fo, ell_o = O.filtering(ys, O.MVNStandard(m0, np.eye(5)), to, oo, OMETH[method], nom_o, True,
+                            return_loglikelihood=True)
+    so = O.smoothing(to, fo, OMETH[method], nom_o, True)
+    nom_d = MVNStandard(t64(nom_m), t64(nom_P))
+    fd, ell_d = parsmooth.filtering(t64(ys), MVNStandard(t64(m0), t64(np.eye(5))), td, od, DMETH[method],
+                                    nom_d, return_loglikelihood=True)
+    sd = parsmooth.smoothing(td, fd, DMETH[method], nom_d)
+    assert isinstance(sd, MVNStandard)
+    close(npy(fd.mean), fo.mean, what="filtered mean")
+    close(npy(fd.cov), fo.cov, what="filtered cov")
+    close(npy(sd.mean), so.mean, what="smoothed mean")
+    close(npy(sd.cov), so.cov, what="smoothed cov")
+    assert abs(float(ell_d) - ell_o) <= TOL64 * max(1.0, abs(ell_o)), (float(ell_d), ell_o)
+
+
+@pytest.mark.parametrize("dim_x,dim_y", [(2, 1), (4, 2)])
+def test_fp32_standard_form_linear(dim_x, dim_y):
+    """FP32 covariance form on a stable linear model, T=333: filtered and smoothed
+    means and covariances against the FP64 oracle run on the same FP32-rounded
+    inputs, within the FP32 tolerances."""
+    r32 = lambda a: np.asarray(a, np.float32).astype(np.float64)
+    x0, _, F, Q, _, b, H, R, _, c, ys = _linear_problem(dim_x, dim_y, 3, 333)
+    F, Q, b, H, R, c, ys = map(r32, (F, Q, b, H, R, c, ys))
+    m0, P0 = r32(x0.mean), r32(x0.cov)
+    nom = r32(np.random.RandomState(5).randn(334, dim_x))
+    nom_P = np.repeat(np.eye(dim_x)[None], 334, 0)
+    to = O.FunctionalModel(O.Linear(F), O.MVNStandard(b, Q))
+    oo = O.FunctionalModel(O.Linear(H), O.MVNStandard(c, R))
+    fo = O.filtering(ys, O.MVNStandard(m0, P0), to, oo, O.extended, O.MVNStandard(nom, nom_P), True)
+    so = O.smoothing(to, fo, O.extended, O.MVNStandard(nom, nom_P), True)
+    t32 = lambda a: torch.as_tensor(np.asarray(a), dtype=torch.float32, device=DEV)
+    td = FunctionalModel(M.LinearFunction(F), MVNStandard(t32(b), t32(Q)))
+    od = FunctionalModel(M.LinearFunction(H), MVNStandard(t32(c), t32(R)))
+    nom_d = MVNStandard(t32(nom), t32(nom_P))
+    fd = parsmooth.filtering(t32(ys), MVNStandard(t32(m0), t32(P0)), td, od, extended, nom_d)
+    sd = parsmooth.smoothing(td, fd, extended, nom_d)
+    assert sd.mean.dtype == torch.float32
+    close(npy(fd.mean), fo.mean, tol=TOL32, what="fp32 std filtered mean")
+    close(npy(fd.cov), fo.cov, tol=TOL32_COV, what="fp32 std filtered cov")
+    close(npy(sd.mean), so.mean, tol=TOL32, what="fp32 std smoothed mean")
+    close(npy(sd.cov), so.cov, tol=TOL32_COV, what="fp32 std smoothed cov")
 
 
 def test_ieks_fixture(golden_dir):
