@@ -228,6 +228,77 @@ def linear_cv_case(steps, warmup):
             "mean_us": mean * 1e6, "median_us": med * 1e6}
 
 
+# Reference-published FP32 stability: % of 100 runs whose FP32 log-likelihood
+# is NaN after 20 iterations (notebooks/robustness_data/R1_rts_fl32_extended_ell.csv
+# and R1_rts_fl32_cubature_ell.csv, protocol notebooks/robustness_100runs.py).
+REF_FP32_NAN_T = [20, 30, 40, 50, 80, 100, 200, 300, 400, 500, 1000, 1500, 2000, 2500, 3000, 3500, 4000,
+                  4500, 5000, 5500, 6000, 6500, 7000, 7500]
+REF_FP32_NAN_PCT = {
+    "extended_sqrt": [0] * 24,
+    "extended_std": [0] * 15 + [1, 0, 1, 1, 1, 1, 1, 2, 2],
+    "cubature_sqrt": [0] * 17 + [1, 0, 0, 0, 1, 2, 2],
+    "cubature_std": [0, 0, 2, 4, 19] + [100] * 19,
+}
+
+
+def fp32_stability(Ts, n_runs=100, n_iter=20):
+    """The reference's FP32 robustness protocol (notebooks/robustness_100runs.py:
+    100 bearings-only runs, each a prefix of length T of one T=8000 simulation;
+    m0 = [-4, -1, 2, 7, 3], P0 = I; nominal [-1, -1, 6, 4, 2] with I at every
+    state; 20 iterations; log-likelihood at the final nominal) in FP32, square-root
+    and covariance form, extended and cubature.  The reference's runs are
+    unseeded; here run j is simulated with seed j.  All runs of one T go into
+    one ragged batch; the log-likelihood accumulates in FP64.  Reports the % of
+    runs with a non-finite log-likelihood next to the reference's published %,
+    and the % of runs whose FP64 iteration (same kernels) diverges -- its final
+    smoothed mean exceeds 1e3 in magnitude or is not finite -- the runs on which
+    FP32 can only fail."""
+    import torch
+    from parsmooth import FunctionalModel, MVNSqrt, MVNStandard
+    from parsmooth.batch import IteratedSmoother
+    from parsmooth.data import bearings_data
+    from parsmooth.linearization import cubature, extended
+    from parsmooth.models import make_parameters
+    dev = torch.device("cuda", 0)
+    Tmax = max(Ts)
+    sims = [bearings_data(np.array([0.1, 0.2, 1.0, 0.0]), 0.01, 0.5, max(Tmax, 8000), (-1.5, 0.5), (1.0, 1.0),
+                          random_state=j + 1)[2] for j in range(n_runs)]
+    Q, R, obs_f, trans_f = make_parameters(0.01, 0.1, 0.5, 0.01, (-1.5, 0.5), (1.0, 1.0))
+    t = lambda a: torch.as_tensor(np.asarray(a, dtype=np.float64)).to(dev, torch.float32)
+    m0 = np.array([-4.0, -1.0, 2.0, 7.0, 3.0])
+    res = {"protocol": "100 runs x 20 iterations, FP32, % runs with non-finite log-likelihood",
+           "T": list(Ts), "ours": {}, "reference": {}, "fp64_diverged": {}}
+    t64 = lambda a: torch.as_tensor(np.asarray(a, dtype=np.float64)).to(dev, torch.float64)
+    for method_name, method in (("extended", extended), ("cubature", cubature)):
+        for form, sqrt in (("sqrt", True), ("std", False)):
+            cls = MVNSqrt if sqrt else MVNStandard
+            key = f"{method_name}_{form}"
+            ours, div64 = [], []
+            for T in Ts:
+                for dt_, tt in ((torch.float32, t), (torch.float64, t64)):
+                    trans = FunctionalModel(trans_f, cls(tt(np.zeros(5)), tt(np.linalg.cholesky(Q) if sqrt else Q)))
+                    obsm = FunctionalModel(obs_f, cls(tt(np.zeros(2)), tt(np.linalg.cholesky(R) if sqrt else R)))
+                    r = IteratedSmoother([T] * n_runs, trans, obsm, method, dtype=dt_, sqrt=sqrt)
+                    r.set_data(tt(np.concatenate([y[:T] for y in sims])), tt(np.tile(m0, (n_runs, 1))),
+                               tt(np.tile(np.eye(5), (n_runs, 1, 1))))
+                    M = r.prob.n_states
+                    init = (tt(np.tile(np.array([-1.0, -1.0, 6.0, 4.0, 2.0]), (M, 1))),
+                            tt(np.tile(np.eye(5), (M, 1, 1))))
+                    mean, _, ell = r.run(n_iter, init, use_graph=False, return_loglikelihood=True)
+                    if dt_ == torch.float32:
+                        ours.append(float(100.0 * np.mean(~np.isfinite(ell.double().cpu().numpy()))))
+                    else:
+                        big = [not bool(torch.isfinite(x).all()) or float(x.abs().max()) > 1e3 for x in r.split(mean)]
+                        div64.append(float(100.0 * np.mean(big)))
+                    del r, init
+            res["ours"][key] = ours
+            res["fp64_diverged"][key] = div64
+            res["reference"][key] = [REF_FP32_NAN_PCT[key][REF_FP32_NAN_T.index(T)] if T in REF_FP32_NAN_T
+                                     else None for T in Ts]
+    torch.cuda.empty_cache()
+    return res
+
+
 def fp32_cubature_errors(exps, n_iter):
     """FP32 square-root cubature IPLS vs the FP64 run of the same kernels (which
     match the FP64 oracle to 1e-9 on the tested sizes) on bearings-only data,
@@ -355,6 +426,8 @@ def main():
                     help="comma-separated log2 T of the IEKS T-sweep ('' to skip)")
     ap.add_argument("--fp32-errors", type=str, default="14,16,18",
                     help="comma-separated log2 T of the FP32 cubature error report ('' to skip)")
+    ap.add_argument("--stability", type=str, default="80,100,1000,7500",
+                    help="T values of the FP32 robustness protocol ('all': the reference's 24; '': skip)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run one 256 MiB device copy first (byte-counter calibration)")
     args = ap.parse_args()
@@ -467,7 +540,7 @@ def main():
                 "errors_over": "the sequences at a fixed point (summed gradient, summed ell)",
                 "grad_fp64": g64.tolist()}
 
-    sweep = lin = errs = None
+    sweep = lin = errs = stab = None
     if args.sweep:
         log(f"T sweep {args.sweep}")
         sweep = t_sweep([int(x) for x in args.sweep.split(",")], n_iter, args.steps, args.warmup)
@@ -475,6 +548,10 @@ def main():
     if args.fp32_errors:
         log(f"FP32 cubature errors {args.fp32_errors}")
         errs = fp32_cubature_errors([int(x) for x in args.fp32_errors.split(",")], 5)
+    if args.stability:
+        log(f"FP32 stability protocol T={args.stability}")
+        Ts = REF_FP32_NAN_T if args.stability == "all" else [int(x) for x in args.stability.split(",")]
+        stab = fp32_stability(Ts)
 
     value = T * n_iter / dt64
     # dominant kernel by total time
@@ -513,6 +590,7 @@ def main():
         "t_sweep_ieks": sweep,
         "linear_cv_T1024": lin,
         "fp32_cubature_error_per_T": errs,
+        "fp32_stability": stab,
         "roofline": roof,
         "cpu_baseline": cpu,
     }

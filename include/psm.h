@@ -86,13 +86,16 @@ typedef struct psm_model {
 } psm_model_t;
 
 /* Unit sigma-point table: points are m + L xi_p.  A NULL table (or npts == 0)
- * selects the extended (Taylor) linearization. */
+ * selects the extended (Taylor) linearization.  The table is FP64 (double)
+ * for every dtype: the regression subtracts F P F^T from the sigma-point
+ * covariance, and FP32-rounded weights (which no longer sum to one) leave a
+ * residual larger than a small process noise. */
 typedef struct psm_sigma {
   int32_t npts;
   int32_t pad_;
-  const void* xi;  /* (npts, n) */
-  const void* wm;  /* (npts,) */
-  const void* wc;  /* (npts,) */
+  const double* xi;  /* (npts, n) */
+  const double* wm;  /* (npts,) */
+  const double* wc;  /* (npts,) */
 } psm_sigma_t;
 
 #define PSM_SCAN_K_MASK 0xff

@@ -150,9 +150,9 @@ inline SigmaView<R> sigma_view(const psm_sigma_t* s, int* use) {
   *use = 0;
   if (s != nullptr && s->npts > 0) {
     v.npts = s->npts;
-    v.xi = static_cast<const R*>(s->xi);
-    v.wm = static_cast<const R*>(s->wm);
-    v.wc = static_cast<const R*>(s->wc);
+    v.xi = static_cast<const double*>(s->xi);
+    v.wm = static_cast<const double*>(s->wm);
+    v.wc = static_cast<const double*>(s->wc);
     *use = 1;
   }
   return v;

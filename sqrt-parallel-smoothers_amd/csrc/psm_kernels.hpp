@@ -130,8 +130,8 @@ struct Lin {
 // ============================================================================
 // 1. linearization at every nominal state
 // ============================================================================
-template <typename R, int NIN, int NOUT, bool SQRT>
-PSM_DEV void linearize_point(const ModelView<R>& md, const SigmaView<R>& sp, int use_sigma,
+template <typename R, int NIN, int NOUT, bool SQRT, typename S>
+PSM_DEV void linearize_point(const ModelView<R, S>& md, const SigmaView<R>& sp, int use_sigma,
                              const R (&m)[NIN], const R (&Lx)[NIN][NIN], R (&F)[NOUT][NIN],
                              R (&C)[NOUT][NOUT], R (&b)[NOUT]) {
   if (use_sigma) {
@@ -147,14 +147,15 @@ PSM_DEV void linearize_point(const ModelView<R>& md, const SigmaView<R>& sp, int
   }
 }
 
-// Thread t fills position t of the state-interleaved buffer (coalesced stores);
-// its state is the inverse of StateMap.
-// Sigma-point linearization of the FP32 square-root path in FP64 arithmetic
-// (FP32 in and out): the statistical linear regression subtracts F P F^T from
-// the sigma-point covariance by rank-one downdates, and when the noise is
-// small against the sigma-point spread (chol Q ~ 6e-5 against a spread ~ 0.1
-// on the bearings-only model) those downdates cancel in FP32 -- the factor
-// loses all its accuracy or the downdate fails outright.
+// Linearization of the FP32 paths in FP64 arithmetic (FP32 in and out).
+// Sigma points: the statistical linear regression subtracts F P F^T from the
+// sigma-point covariance by rank-one downdates, and when the noise is small
+// against the sigma-point spread (chol Q ~ 6e-5 against a spread ~ 0.1 on the
+// bearings-only model) those downdates cancel in FP32 -- the factor loses all
+// its accuracy or the downdate fails outright.  Extended: model Jacobians
+// such as d/dw (1 - cos(w dt)) / w of the coordinated turn cancel in FP32 at
+// small turn rates (cos(w dt) rounds to 1 and the derivative doubles), and
+// b = f(m) - F m cancels at large |m|.
 template <typename R> struct WiderOf { using type = R; };
 template <> struct WiderOf<float> { using type = double; };
 template <> struct WiderOf<DualT<float>> { using type = DualT<double>; };
@@ -175,23 +176,23 @@ PSM_DEV void linearize_point_w(const ModelView<R>& md, const SigmaView<R>& sp, i
                                const R (&m)[NIN], const R (&Lx)[NIN][NIN], R (&F)[NOUT][NIN],
                                R (&C)[NOUT][NOUT], R (&b)[NOUT]) {
   using W = typename WiderOf<R>::type;
-  if constexpr (SQRT && !std::is_same<W, R>::value) {
-    if (use_sigma) {
-      const ModelView<W, R> mw{md.kind, md.params, md.q_mean, md.q_chol, md.row, md.pt, md.npts};
-      const SigmaView<W, R> sw{sp.npts, sp.xi, sp.wm, sp.wc};
-      W mW[NIN], LW[NIN][NIN], FW[NOUT][NIN], CW[NOUT][NOUT], bW[NOUT];
-      widen(m, mW);
-      widen(Lx, LW);
-      linearize_sigma<W, NIN, NOUT, true>(mw, sw, mW, LW, FW, CW, bW);
-      widen(FW, F);
-      widen(CW, C);
-      widen(bW, b);
-      return;
-    }
+  if constexpr (!std::is_same<W, R>::value) {
+    const ModelView<W, R> mw{md.kind, md.params, md.q_mean, md.q_chol, md.row, md.pt, md.npts};
+    const SigmaView<W> sw{sp.npts, sp.xi, sp.wm, sp.wc};
+    W mW[NIN], LW[NIN][NIN], FW[NOUT][NIN], CW[NOUT][NOUT], bW[NOUT];
+    widen(m, mW);
+    widen(Lx, LW);
+    linearize_point<W, NIN, NOUT, SQRT>(mw, sw, use_sigma, mW, LW, FW, CW, bW);
+    widen(FW, F);
+    widen(CW, C);
+    widen(bW, b);
+  } else {
+    linearize_point<R, NIN, NOUT, SQRT>(md, sp, use_sigma, m, Lx, F, C, b);
   }
-  linearize_point<R, NIN, NOUT, SQRT>(md, sp, use_sigma, m, Lx, F, C, b);
 }
 
+// Thread t fills position t of the state-interleaved buffer (coalesced stores);
+// its state is the inverse of StateMap.
 template <typename R, int NX, int NY, bool SQRT>
 __global__ __launch_bounds__(kLaneWG) void k_linearize_states(
     int64_t M, int64_t Mp, int C, ModelView<R> tr, ModelView<R> ob, int do_obs, SigmaView<R> sp,

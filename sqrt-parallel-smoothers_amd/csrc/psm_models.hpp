@@ -193,8 +193,13 @@ PSM_DEV void model_value(const ModelView<R, S>& md, const R (&x)[NIN], R (&y)[NO
   model_eval<R, NIN, NOUT, false>(md, x, y, J);
 }
 
-// Sigma-point table (unit points xi_p, weights) in device memory.
-template <typename R, typename S = R>
+// Sigma-point table (unit points xi_p, weights) in device memory, stored in
+// FP64 for every compute type: the statistical linear regression subtracts
+// F P F^T from the sigma-point covariance, and weights rounded to FP32 (1/10
+// is 0.1000000015) no longer sum to one, which leaves a residual of ~1e-8 of
+// the covariance -- more than a small process noise (3e-9 on the
+// bearings-only model) -- in the regression noise factor.
+template <typename R, typename S = double>
 struct SigmaView {
   int npts;
   const S* xi;  // npts x n
@@ -240,7 +245,7 @@ PSM_DEV void linearize_extended(const ModelView<R, S>& md, const R (&m)[NIN], R 
 // SQRT=false returns Phi - F P F^T + Q symmetrised (:80-85).
 // ----------------------------------------------------------------------------
 template <typename R, int NIN, int NOUT, bool SQRT, typename S>
-PSM_DEV void linearize_sigma(const ModelView<R, S>& md, const SigmaView<R, S>& sp, const R (&m)[NIN],
+PSM_DEV void linearize_sigma(const ModelView<R, S>& md, const SigmaView<R>& sp, const R (&m)[NIN],
                              const R (&L)[NIN][NIN], R (&F)[NOUT][NIN], R (&C)[NOUT][NOUT],
                              R (&b)[NOUT]) {
   R mf[NOUT];

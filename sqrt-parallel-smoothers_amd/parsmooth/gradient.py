@@ -99,14 +99,9 @@ class IteratedLoglikelihood:
 
     # -------------------------------------------------------------- passes
     def _sigma(self, tangent):
-        s, keep = self.method.sigma(self.nx, self.dtype, self.dev)
-        if s is None:
-            return None, None
-        if not tangent:
-            return s, keep
-        dual = [_dual(t, torch.zeros_like(t)) for t in keep]
-        sd = N.PsmSigma(s.npts, 0, dual[0].data_ptr(), dual[1].data_ptr(), dual[2].data_ptr())
-        return sd, dual
+        # the sigma table is FP64 for value and tangent passes alike (constant:
+        # no tangent)
+        return self.method.sigma(self.nx, self.dtype, self.dev)
 
     def _filter_smoother(self, prob, models, sigma, ys, m0, L0, nom, out, filt):
         rc = N.lib().psm_filter_smoother(

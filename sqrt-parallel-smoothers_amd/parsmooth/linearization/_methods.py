@@ -35,14 +35,16 @@ class Method:
         return self.table_fn is not None
 
     def sigma(self, n, dtype, dev):
-        """ctypes PsmSigma for state dimension n (device tensors cached)."""
+        """ctypes PsmSigma for state dimension n (device tensors cached).  The
+        table is FP64 whatever the compute dtype: FP32-rounded weights would not
+        sum to one, which the regression's F P F^T subtraction cannot absorb."""
         if self.table_fn is None:
             return None, None
-        key = (n, dtype, str(dev))
+        key = (n, str(dev))
         hit = self._cache.get(key)
         if hit is None:
             xi, wm, wc = self.table_fn(n, **self.kwargs)
-            t = [torch.as_tensor(np.ascontiguousarray(a)).to(device=dev, dtype=dtype) for a in (xi, wm, wc)]
+            t = [torch.as_tensor(np.ascontiguousarray(a, dtype=np.float64)).to(device=dev) for a in (xi, wm, wc)]
             s = N.PsmSigma(xi.shape[0], 0, t[0].data_ptr(), t[1].data_ptr(), t[2].data_ptr())
             hit = (s, t)
             self._cache[key] = hit

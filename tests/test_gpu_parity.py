@@ -545,6 +545,64 @@ def test_fp32_bearings_vs_fp64_oracle():
     close(npy(got.mean), ref.mean, tol=1e-4, what="fp32 mean")
 
 
+def test_fp32_extended_small_turn_rate():
+    """FP32 extended linearization of the coordinated turn at small turn rates
+    (w dt from 1e-4 down to 2e-8, above the model's 1e-6 branch on w): the
+    turn-rate column of the Jacobian, e.g. d/dw (1 - cos(w dt)) / w, cancels in
+    FP32 arithmetic (cos(w dt) rounds to 1 and the derivative doubles), so FP32
+    problems linearize in FP64; each entry must match the FP64 oracle to 1e-5 of
+    its own size, and b = f(m) - F m to FP32 rounding of a large |m|."""
+    dt = float(np.float32(0.01))
+    Q, _, _, trans_f = M.make_parameters(0.01, 0.1, 0.5, dt, (-1.5, 0.5), (1.0, 1.0))
+    cQ = np.asarray(np.linalg.cholesky(Q), np.float32).astype(np.float64)
+    rng = np.random.RandomState(12)
+    ws = np.array([1e-2, -1.7e-3, 4e-4, -1e-4, 1e-5, 2.1e-6])
+    m = np.zeros((len(ws), 5))
+    m[:, :4] = 3e3 * rng.randn(len(ws), 4)
+    m[:, 4] = ws
+    m = np.asarray(m, np.float32).astype(np.float64)
+    L = np.repeat(np.eye(5)[None], len(ws), 0)
+    t32 = lambda a: torch.as_tensor(np.asarray(a), dtype=torch.float32, device=DEV)
+    F, _, b = [npy(g) for g in extended(FunctionalModel(trans_f, MVNSqrt(t32(np.zeros(5)), t32(cQ))),
+                                        MVNSqrt(t32(m), t32(L)))]
+    Fo, _, bo = O.extended(O.FunctionalModel(O.CoordinatedTurn(dt), O.MVNSqrt(np.zeros(5), cQ)), O.MVNSqrt(m, L))
+    col, colo = F[:, :, 4], Fo[:, :, 4]
+    assert np.all(np.abs(col - colo) <= 1e-5 * np.abs(colo) + 1e-12), (col, colo)
+    close(F, Fo, tol=1e-6, what="fp32 CT Jacobian")
+    close(b, bo, tol=1e-6, what="fp32 CT offset")
+
+
+@pytest.mark.parametrize("method", ["cubature", "unscented", "gauss_hermite"])
+def test_fp32_sigma_regression_small_noise(method):
+    """FP32 sigma-point linearization of the coordinated-turn transition, whose
+    process noise (chol ~6e-5 on the positions) is tiny against the sigma-point
+    spread (0.3): the regression noise factor chol(Phi + Q - F P F^T) against the
+    FP64 oracle on the same FP32-rounded inputs.  The cancellation of Phi and
+    F P F^T is exact only with weights that sum to one and points with
+    sum w xi xi^T = I, which an FP32-rounded sigma table breaks by ~1e-8 -- more
+    than Q itself -- so the table is FP64 and this factor must match."""
+    r32 = lambda a: np.asarray(a, np.float32).astype(np.float64)
+    dt = float(np.float32(0.01))
+    Q, _, _, trans_f = M.make_parameters(0.01, 0.1, 0.5, dt, (-1.5, 0.5), (1.0, 1.0))
+    cQ = r32(np.linalg.cholesky(Q))
+    rng = np.random.RandomState(11)
+    B = 64
+    m = r32(np.array([0.1, 0.2, 1.0, 0.0, 0.5]) + rng.randn(B, 5))
+    L = r32(np.stack([0.3 * np.eye(5) + 0.05 * np.tril(rng.randn(5, 5), -1) for _ in range(B)]))
+    t32 = lambda a: torch.as_tensor(np.asarray(a), dtype=torch.float32, device=DEV)
+    got = [npy(g) for g in DMETH[method](FunctionalModel(trans_f, MVNSqrt(t32(np.zeros(5)), t32(cQ))),
+                                         MVNSqrt(t32(m), t32(L)))]
+    ref = OMETH[method](O.FunctionalModel(O.CoordinatedTurn(dt), O.MVNSqrt(np.zeros(5), cQ)), O.MVNSqrt(m, L))
+    close(got[0], ref[0], tol=TOL32, what=f"fp32 {method} F")
+    close(got[2], ref[2], tol=TOL32, what=f"fp32 {method} b")
+    # the noise factor relative to its own scale (Q ~ 1e-9 .. 1e-3)
+    Pg, Pr = gram(got[1]), gram(ref[1])
+    err = float(np.abs(Pg - Pr).max() / np.abs(Pr).max())
+    assert err <= 1e-3, f"fp32 {method} regression noise: rel err {err:.2e}"
+    dg, dr = np.abs(np.diagonal(got[1], axis1=1, axis2=2)), np.abs(np.diagonal(ref[1], axis1=1, axis2=2))
+    assert np.all(np.abs(dg - dr) <= 1e-3 * dr + 1e-9), "fp32 regression noise factor diagonal"
+
+
 @pytest.mark.parametrize("k", [2, 2 | N.SCAN_TEAM, 2 | N.SCAN_TEAM | N.SCAN_HALF, 4 | N.SCAN_HALF])
 def test_fp32_schedules_match_oracle(k):
     """FP32 filter-smoother pass over a ragged batch with per-lane, team-cooperative
