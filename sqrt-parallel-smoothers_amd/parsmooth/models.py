@@ -173,6 +173,13 @@ def is_device_function(f) -> bool:
     return isinstance(f, DeviceFunction)
 
 
+def _eval_dtype(dtype):
+    """Host evaluation dtype of torch-callable models: FP32 problems evaluate in
+    FP64 and round the table (as the kernels linearize FP32 problems in FP64:
+    Jacobians such as d/dw (1 - cos(w dt)) / w cancel in FP32)."""
+    return torch.float64 if dtype == torch.float32 else dtype
+
+
 class TorchFunction(DeviceFunction):
     """A model function evaluated by torch on the host side of the boundary.
 
@@ -210,6 +217,8 @@ class TorchFunction(DeviceFunction):
         xi None -> [f(m), J(m)] (n, nout + nout nin); else f(m + L xi_p) (n, npts nout).
         ``out`` (n, row width), when given, receives the rows in place."""
         n = m.shape[0]
+        out_dtype = m.dtype
+        m, L = m.to(_eval_dtype(out_dtype)), L.to(_eval_dtype(out_dtype))
         if xi is None:
             f1 = lambda v: self._eval(v.reshape(1, self.nin)).reshape(self.nout)
             y = self._eval(m)
@@ -219,7 +228,7 @@ class TorchFunction(DeviceFunction):
             xi = xi.to(dtype=m.dtype, device=m.device)
             chi = m[:, None, :] + torch.einsum("nij,pj->npi", L, xi)
             rows = self._eval(chi.reshape(-1, self.nin)).reshape(n, xi.shape[0] * self.nout)
-        rows = rows.to(m.dtype)
+        rows = rows.to(out_dtype)
         if out is None:
             return rows.contiguous()
         out.copy_(rows)
@@ -263,6 +272,8 @@ class ConditionalMoments(TorchFunction):
 
     def tabulate(self, m: torch.Tensor, L: torch.Tensor, xi, out: torch.Tensor = None):
         n = m.shape[0]
+        out_dtype = m.dtype
+        m, L = m.to(_eval_dtype(out_dtype)), L.to(_eval_dtype(out_dtype))
         if xi is None:
             f1 = lambda v: torch.as_tensor(self.fn(v), dtype=v.dtype, device=v.device).reshape(self.nout)
             J = torch.func.vmap(torch.func.jacfwd(f1))(m)
@@ -271,7 +282,7 @@ class ConditionalMoments(TorchFunction):
             xi = xi.to(dtype=m.dtype, device=m.device)
             chi = (m[:, None, :] + torch.einsum("nij,pj->npi", L, xi)).reshape(-1, self.nin)
             rows = torch.cat([self._mean(chi), self._factor(chi)], dim=1).reshape(n, -1)
-        rows = rows.to(m.dtype)
+        rows = rows.to(out_dtype)
         if out is None:
             return rows.contiguous()
         out.copy_(rows)

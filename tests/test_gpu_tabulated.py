@@ -71,6 +71,34 @@ def test_linearize_torch_callable_matches_builtin(method):
             close(npy(v), npy(u), tol if method != "extended" else 1e-12, f"{method} {name}")
 
 
+def test_fp32_torch_callable_small_turn_rate():
+    """An FP32 problem evaluates torch callables in FP64 (and rounds the table),
+    like the kernels' own models: the coordinated-turn Jacobian's turn-rate
+    column, which cancels in FP32 at small w, matches the FP64 oracle entry by
+    entry to 1e-5 of its size, and the regression noise of a cubature
+    linearization matches it too."""
+    dt = float(np.float32(0.01))
+    Q, _, _, _ = M.make_parameters(0.01, 0.1, 0.5, dt, (-1.5, 0.5), (1.0, 1.0))
+    cQ = np.asarray(np.linalg.cholesky(Q), np.float32).astype(np.float64)
+    rng = np.random.RandomState(13)
+    ws = np.array([1e-2, -1.7e-3, 4e-4, -1e-4, 1e-5, 2.1e-6])
+    m = np.zeros((len(ws), 5))
+    m[:, :4] = 5.0 * rng.randn(len(ws), 4)
+    m[:, 4] = ws
+    m = np.asarray(m, np.float32).astype(np.float64)
+    L = np.repeat(0.3 * np.eye(5)[None], len(ws), 0)
+    t32 = lambda a: torch.as_tensor(np.asarray(a), dtype=torch.float32, device=DEV)
+    model = FunctionalModel(ct_torch(dt), MVNSqrt(t32(np.zeros(5)), t32(cQ)))
+    om = O.FunctionalModel(O.CoordinatedTurn(dt), O.MVNSqrt(np.zeros(5), cQ))
+    F, _, _ = [npy(g) for g in extended(model, MVNSqrt(t32(m), t32(L)))]
+    Fo, _, _ = O.extended(om, O.MVNSqrt(m, L))
+    assert np.all(np.abs(F[:, :, 4] - Fo[:, :, 4]) <= 1e-5 * np.abs(Fo[:, :, 4]) + 1e-12)
+    _, C, _ = [npy(g) for g in cubature(model, MVNSqrt(t32(m), t32(L)))]
+    _, Co, _ = O.cubature(om, O.MVNSqrt(m, L))
+    dg, do = np.abs(np.diagonal(C, axis1=1, axis2=2)), np.abs(np.diagonal(Co, axis1=1, axis2=2))
+    assert np.all(np.abs(dg - do) <= 1e-3 * do + 1e-9), (dg, do)
+
+
 @pytest.mark.parametrize("method", ["extended", "cubature"])
 def test_iterated_smoothing_with_torch_callables(method):
     """The drop-in call with plain callables in FunctionalModel gives the same
