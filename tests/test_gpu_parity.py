@@ -266,6 +266,91 @@ def test_filter_smoother_linear(dim_x, dim_y, sqrt, T):
         close(npy(sd[1]), so[1], tol=1e-8, what="smoothed chol")
 
 
+def _linear_models(F, b, q, H, c, r, sqrt):
+    cls = MVNSqrt if sqrt else MVNStandard
+    return (FunctionalModel(M.LinearFunction(F), cls(t64(b), t64(q))),
+            FunctionalModel(M.LinearFunction(H), cls(t64(c), t64(r))), cls)
+
+
+@pytest.mark.parametrize("dim_x,dim_y", [(1, 2), (3, 2), (3, 3)])
+@pytest.mark.parametrize("seed", [0, 42])
+@pytest.mark.parametrize("sqrt", [True, False])
+@pytest.mark.parametrize("method", list(DMETH))
+def test_filter_no_information(dim_x, dim_y, seed, sqrt, method):
+    """tests/test_sequential_filter.py:120-150 through the parallel filter:
+    observation noise x1e12 (chol x1e6), the filtered means are the prediction
+    F m + b iterated from the prior (rtol/atol 1e-3, the reference's)."""
+    np.random.seed(seed)
+    T = 3
+    x0, cx0, F, Q, cholQ, b, _ = O.get_system(dim_x, dim_x)
+    _, _, H, R, cholR, c, _ = O.get_system(dim_x, dim_y)
+    R, cholR = 1e12 * R, 1e6 * cholR
+    _, ys = O.lgssm_data(x0.mean, F, H, R, Q, b, c, T, random_state=seed)
+    td, od, cls = _linear_models(F, b, cholQ if sqrt else Q, H, c, cholR if sqrt else R, sqrt)
+    prior = cls(t64(x0.mean), t64(cx0.chol if sqrt else x0.cov))
+    f = parsmooth.filtering(t64(ys), prior, td, od, DMETH[method])
+    expected = [x0.mean]
+    for _ in range(T):
+        expected.append(F @ expected[-1] + b)
+    np.testing.assert_allclose(npy(f.mean), np.stack(expected), atol=1e-3, rtol=1e-3)
+
+
+@pytest.mark.parametrize("dim", [1, 3])
+@pytest.mark.parametrize("seed", [0, 42])
+@pytest.mark.parametrize("sqrt", [True, False])
+@pytest.mark.parametrize("method", list(DMETH))
+def test_filter_infinite_information(dim, seed, sqrt, method):
+    """tests/test_sequential_filter.py:153-184 through the parallel filter: F = H
+    = I, observation noise x1e-6 (chol x1e-3): the filtered means are y - c."""
+    np.random.seed(seed)
+    T = 3
+    x0, cx0, _, Q, cholQ, b, _ = O.get_system(dim, dim)
+    F = np.eye(dim)
+    _, _, _, R, cholR, c, _ = O.get_system(dim, dim)
+    H = np.eye(dim)
+    R, cholR = 1e-6 * R, 1e-3 * cholR
+    _, ys = O.lgssm_data(x0.mean, F, H, R, Q, b, c, T, random_state=seed)
+    td, od, cls = _linear_models(F, b, cholQ if sqrt else Q, H, c, cholR if sqrt else R, sqrt)
+    prior = cls(t64(x0.mean), t64(cx0.chol if sqrt else x0.cov))
+    f = parsmooth.filtering(t64(ys), prior, td, od, DMETH[method])
+    np.testing.assert_allclose(npy(f.mean)[1:], np.asarray(ys) - c, atol=1e-3, rtol=1e-3)
+
+
+@pytest.mark.parametrize("dim_x", [1, 3])
+@pytest.mark.parametrize("seed", [0, 42])
+@pytest.mark.parametrize("sqrt", [True, False])
+@pytest.mark.parametrize("noise", ["none", "infinite"])
+def test_smoother_step_noise_limits(dim_x, seed, sqrt, noise):
+    """tests/test_sequential_smoother.py:67-120 through the parallel smoother:
+    smoothing the two-state filtered trajectory [x_f, x_s] gives at state 0 the
+    smoothing step from x_f given the smoothed x_s.  No process noise: the RTS
+    step with gain F^+ (mean x_f + F^+ (m_s - F m_f - b), covariance
+    P_f + F^+ (P_s - F P_f F^T) F^+T); infinite process noise: x_f unchanged
+    (atol 1e-5, the reference's)."""
+    np.random.seed(seed)
+    xf, cxf, F, Q, cholQ, b, _ = O.get_system(dim_x, dim_x)
+    xs, cxs, *_ = O.get_system(dim_x, dim_x)
+    if noise == "none":
+        Q, cholQ = 0.0 * Q, 0.0 * cholQ
+    else:
+        Q, cholQ = 1e12 * Q, 1e6 * cholQ
+    cls = MVNSqrt if sqrt else MVNStandard
+    model = FunctionalModel(M.LinearFunction(F), cls(t64(b), t64(cholQ if sqrt else Q)))
+    traj = cls(t64(np.stack([xf.mean, xs.mean])),
+               t64(np.stack([cxf.chol, cxs.chol]) if sqrt else np.stack([xf.cov, xs.cov])))
+    sm = parsmooth.smoothing(model, traj, extended)
+    m0 = npy(sm.mean)[0]
+    P0 = gram(npy(sm.chol)[0]) if sqrt else npy(sm.cov)[0]
+    if noise == "none":
+        G = np.linalg.pinv(F)
+        ms = xf.mean + G @ (xs.mean - (F @ xf.mean + b))
+        Ps = xf.cov + G @ (xs.cov - F @ xf.cov @ F.T) @ G.T
+    else:
+        ms, Ps = xf.mean, xf.cov
+    np.testing.assert_allclose(m0, ms, atol=1e-5)
+    np.testing.assert_allclose(P0, Ps, atol=1e-5)
+
+
 def _bearings_setup(T, seed=0):
     Q, R, obs_f, trans_f = M.make_parameters(0.01, 0.1, 0.5, 0.01, (-1.5, 0.5), (1.0, 1.0))
     from parsmooth.data import bearings_data
