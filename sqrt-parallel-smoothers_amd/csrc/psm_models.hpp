@@ -193,6 +193,10 @@ PSM_DEV void model_value(const ModelView<R, S>& md, const R (&x)[NIN], R (&y)[NO
   model_eval<R, NIN, NOUT, false>(md, x, y, J);
 }
 
+// Sigma points whose regression columns join the square-root factor in one
+// triangularisation.
+constexpr int kSigmaBatch = 4;
+
 // Sigma-point table (unit points xi_p, weights) in device memory, stored in
 // FP64 for every compute type: the statistical linear regression subtracts
 // F P F^T from the sigma-point covariance, and weights rounded to FP32 (1/10
@@ -291,7 +295,8 @@ PSM_DEV void linearize_sigma(const ModelView<R, S>& md, const SigmaView<R>& sp, 
   } else {
     set_zero(Phi);
   }
-  for (int p = 0; p < sp.npts; ++p) {
+  // sigma point p: deviation d = f(chi_p) - mf, weight wc_p; accumulates Psi
+  auto point = [&](int p, R (&d)[NOUT], R& w) {
     R chi[NIN], dx[NIN];
 #pragma unroll
     for (int i = 0; i < NIN; ++i) {
@@ -305,8 +310,7 @@ PSM_DEV void linearize_sigma(const ModelView<R, S>& md, const SigmaView<R>& sp, 
     ModelView<R, S> mp = md;
     mp.pt = p;
     model_value<R, NIN, NOUT>(mp, chi, f);
-    const R w = sp.pwc(p);
-    R d[NOUT];
+    w = sp.pwc(p);
 #pragma unroll
     for (int i = 0; i < NOUT; ++i) d[i] = f[i] - mf[i];
 #pragma unroll
@@ -315,6 +319,44 @@ PSM_DEV void linearize_sigma(const ModelView<R, S>& md, const SigmaView<R>& sp, 
 #pragma unroll
       for (int j = 0; j < NOUT; ++j) Psi[i][j] = fma(wx, d[j], Psi[i][j]);
     }
+  };
+  bool batched = false;
+  if constexpr (SQRT) {
+    if (md.kind != PSM_MODEL_CONDITIONAL) {
+      // the columns sqrt(wc_p) d_p join the factor kSigmaBatch at a time: one
+      // triangularisation of [factor | columns] per batch instead of one per
+      // point (the reference triangularises all columns at once); a batch
+      // past the last point is padded with zero columns
+      batched = true;
+      for (int p0 = 0; p0 < sp.npts; p0 += kSigmaBatch) {
+        R Ab[NOUT][NOUT + kSigmaBatch];
+#pragma unroll
+        for (int i = 0; i < NOUT; ++i)
+#pragma unroll
+          for (int j = 0; j < NOUT; ++j) Ab[i][j] = Acc[i][j];
+#pragma unroll
+        for (int k = 0; k < kSigmaBatch; ++k) {
+          R d[NOUT], w = R(0);
+          set_zero(d);
+          const bool live = p0 + k < sp.npts;
+          if (live) point(p0 + k, d, w);
+          const R sw = live ? sqrt_(w) : R(0);
+#pragma unroll
+          for (int i = 0; i < NOUT; ++i) Ab[i][NOUT + k] = sw * d[i];
+        }
+        tria_inplace<R, NOUT, NOUT + kSigmaBatch>(Ab);
+#pragma unroll
+        for (int i = 0; i < NOUT; ++i)
+#pragma unroll
+          for (int j = 0; j < NOUT; ++j) Acc[i][j] = Ab[i][j];
+      }
+    }
+  }
+  for (int p = 0; p < sp.npts && !batched; ++p) {
+    R d[NOUT], w;
+    point(p, d, w);
+    ModelView<R, S> mp = md;
+    mp.pt = p;
     if constexpr (SQRT) {
       const R sw = sqrt_(w);
 #pragma unroll
