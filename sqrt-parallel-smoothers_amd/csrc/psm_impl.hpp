@@ -1,6 +1,8 @@
 // Host-side launchers for one (real type, nx, ny, form) instantiation, and the
 // function table the C ABI dispatches through.
 #pragma once
+#include <cerrno>
+#include <climits>
 #include <cstdio>
 #include <cstdlib>
 #include <string>
@@ -11,7 +13,7 @@
 namespace psm {
 
 // ---------------------------------------------------------------- errors
-void set_error(const std::string& msg);   // defined in psm_api.cpp
+void set_error(const std::string& msg);   // defined in psm_api.hip
 
 inline int hip_check(const char* what) {
   hipError_t e = hipGetLastError();
@@ -59,23 +61,27 @@ struct Plan {
   size_t off_lin = 0, off_ell = 0, off_selem = 0, total = 0;
 };
 
-inline int env_int(const char* name, int dflt) {
+// Integer environment setting within [lo, hi]; unset, empty, malformed or
+// out-of-range values give dflt.
+inline int env_int(const char* name, int dflt, int lo = INT_MIN, int hi = INT_MAX) {
   const char* v = std::getenv(name);
-  return v ? std::atoi(v) : dflt;
+  if (v == nullptr || *v == '\0') return dflt;
+  char* end = nullptr;
+  errno = 0;
+  const long x = std::strtol(v, &end, 10);
+  if (errno != 0 || end == v || *end != '\0' || x < lo || x > hi) return dflt;
+  return (int)x;
 }
 
 inline size_t align_up(size_t x) { return (x + 255) & ~size_t(255); }
 
-// Default chunk (states per lane in the reduce / apply passes): small enough to
-// keep every CU busy on a single long sequence, larger when the batch is big.
 // Default states per chunk: 4, or larger so that the chunk aggregates fit one
 // round of 64-lane, fan-in-2 level-0 workgroups on 256 CUs (32768 items; one
-// aggregate more would run a nearly empty workgroup in a second round); 16 in
-// the throughput regime.
+// aggregate more would run a nearly empty workgroup in a second round), at
+// most 16 (the throughput regime's chunk).
 inline int default_chunk(int64_t M) {
-  if (M >= (int64_t)1 << 22) return 16;
   const int64_t c = (M + 32767) / 32768;
-  return c < 4 ? 4 : (int)c;
+  return c < 4 ? 4 : (c > 16 ? 16 : (int)c);
 }
 
 // wg: lanes per workgroup of level 0 (one lane per combination); wgu: lanes or
@@ -86,7 +92,7 @@ inline Plan make_plan(const psm_problem_t* p, size_t rsize, int nlin, int nqf, i
   if (wgu <= 0) wgu = wg;
   Plan pl;
   pl.M = p->n_states;
-  pl.chunk = p->chunk > 0 ? p->chunk : env_int("PSM_CHUNK", default_chunk(pl.M));
+  pl.chunk = p->chunk > 0 ? p->chunk : env_int("PSM_CHUNK", default_chunk(pl.M), 1, 4096);
   pl.nchunks = (pl.M + pl.chunk - 1) / pl.chunk;
   pl.Mp = state_stride(pl.M, pl.chunk);
   pl.wg = wg;
@@ -99,7 +105,7 @@ inline Plan make_plan(const psm_problem_t* p, size_t rsize, int nlin, int nqf, i
   pl.off_selem = off;
   off = align_up(off + (size_t)nqs * pl.Mp * rsize);
   const int kreq = p->scan_k & PSM_SCAN_K_MASK;
-  const int kmax = kreq > 0 ? kreq : env_int("PSM_SCAN_K", kdef);
+  const int kmax = kreq > 0 ? kreq : env_int("PSM_SCAN_K", kdef, 1, PSM_SCAN_K_MASK);
   const int ktop_req = (p->scan_k & PSM_SCAN_TOP_MASK) >> PSM_SCAN_TOP_SHIFT;
   const int ktop = ktop_req > kmax ? ktop_req : kmax;
   int64_t n = pl.nchunks;

@@ -83,6 +83,23 @@ def test_workspace_bytes_and_argument_errors():
     assert rc == -2 and b"unsupported" in lib.psm_last_error()
 
 
+def test_schedule_environment_is_validated(monkeypatch):
+    """PSM_CHUNK / PSM_SCAN_K that are empty, malformed or out of range fall back
+    to the defaults instead of reaching the planner (chunk 0 would divide by
+    zero)."""
+    lib = N.load_library()
+    p = N.PsmProblem(N.PSM_F64, N.FORM_SQRT, 5, 2, 1, 0, 131073, 1)
+    want = lib.psm_chunk_size(ctypes.byref(p))
+    ws = lib.psm_workspace_bytes(ctypes.byref(p))
+    for bad in ("", "abc", "0", "-3", "4x", "99999999999"):
+        monkeypatch.setenv("PSM_CHUNK", bad)
+        monkeypatch.setenv("PSM_SCAN_K", bad)
+        assert lib.psm_chunk_size(ctypes.byref(p)) == want, bad
+        assert lib.psm_workspace_bytes(ctypes.byref(p)) == ws, bad
+    monkeypatch.setenv("PSM_CHUNK", "7")
+    assert lib.psm_chunk_size(ctypes.byref(p)) == 7
+
+
 def test_schedule_fields_change_the_plan():
     """chunk and scan_k of the problem select the launch schedule; the workspace
     follows it (interleaved buffers are padded to whole 64-chunk groups)."""
@@ -96,6 +113,10 @@ def test_schedule_fields_change_the_plan():
             sizes[chunk, k] = lib.psm_workspace_bytes(ctypes.byref(p))
             assert sizes[chunk, k] > 100001 * 8 * (71 + 56)
     assert len(set(sizes.values())) > 1
+    # default chunk: 4, larger to keep the aggregates within one round (<= 32768), at most 16
+    for M, want in ((1025, 4), (131072, 4), (131073, 5), (262145, 9), (1 << 21, 16), (1 << 24, 16)):
+        p = N.PsmProblem(N.PSM_F64, N.FORM_SQRT, 5, 2, 1, 0, M, 1)
+        assert lib.psm_chunk_size(ctypes.byref(p)) == want, (M, lib.psm_chunk_size(ctypes.byref(p)))
     # a larger top-level fold limit can only remove levels (less workspace)
     p = N.PsmProblem(N.PSM_F64, N.FORM_SQRT, 5, 2, 1, 4, 4093, 1)  # 1024 chunk aggregates
     p.scan_k = 2
