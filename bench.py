@@ -86,18 +86,22 @@ def make_inputs(T, seed=0):
     return ys
 
 
-def build_runner(T, ys, dtype, n_seq=1, method=None):
+def build_runner(T, ys, dtype, n_seq=1, method=None, sqrt=True):
+    """Bearings-only IEKS/IPLS runner; square-root (MVNSqrt) or covariance
+    (MVNStandard, sqrt=False) form.  The initial nominal is the notebook's
+    [-1, -1, 6, 4, 2] with identity chol / covariance at every state."""
     import torch
-    from parsmooth import FunctionalModel, MVNSqrt
+    from parsmooth import FunctionalModel, MVNSqrt, MVNStandard
     from parsmooth.batch import IteratedSmoother
     from parsmooth.linearization import extended
     from parsmooth.models import make_parameters
     dev = torch.device("cuda", 0)
     Q, R, obs_f, trans_f = make_parameters(0.01, 0.1, 0.5, 0.01, (-1.5, 0.5), (1.0, 1.0))
     t = lambda a: torch.as_tensor(np.asarray(a, dtype=np.float64)).to(dev, dtype)
-    trans = FunctionalModel(trans_f, MVNSqrt(t(np.zeros(5)), t(np.linalg.cholesky(Q))))
-    obs = FunctionalModel(obs_f, MVNSqrt(t(np.zeros(2)), t(np.linalg.cholesky(R))))
-    runner = IteratedSmoother([T] * n_seq, trans, obs, method or extended, dtype=dtype)
+    cls = MVNSqrt if sqrt else MVNStandard
+    trans = FunctionalModel(trans_f, cls(t(np.zeros(5)), t(np.linalg.cholesky(Q) if sqrt else Q)))
+    obs = FunctionalModel(obs_f, cls(t(np.zeros(2)), t(np.linalg.cholesky(R) if sqrt else R)))
+    runner = IteratedSmoother([T] * n_seq, trans, obs, method or extended, dtype=dtype, sqrt=sqrt)
     m0 = np.tile(np.array([0.1, 0.2, 1.0, 0.0, 0.0]), (n_seq, 1))
     runner.set_data(t(np.tile(ys, (n_seq, 1))), t(m0), t(np.tile(np.eye(5), (n_seq, 1, 1))))
     M = runner.prob.n_states
@@ -239,6 +243,41 @@ REF_FP32_NAN_PCT = {
     "cubature_sqrt": [0] * 17 + [1, 0, 0, 0, 1, 2, 2],
     "cubature_std": [0, 0, 2, 4, 19] + [100] * 19,
 }
+
+
+# Reference-published GPU times at T=4096 (BASELINE 1.1 / 1.2, from
+# notebooks/new_outputs/rtx-GPU-all-methods.csv and notebooks/outputs/GPU-all-methods
+# line 11): mean wall time per call of iterated_smoothing, 10 iterations, FP64.
+REF_T4096_MS = {
+    "extended_sqrt": {"rtx_3080_ti": 301.2, "tesla": 298.3},
+    "cubature_sqrt": {"rtx_3080_ti": 375.1, "tesla": 272.5},
+    "extended_cov": {"rtx_3080_ti": 68.5, "tesla": 74.4},
+    "cubature_cov": {"rtx_3080_ti": 86.1, "tesla": 64.7},
+}
+
+
+def published_cases(steps, warmup):
+    """The reference's own GPU benchmark cases at T=4096 (bearings-only, 10
+    iterations, FP64, nominal [-1, -1, 6, 4, 2] / I; the notebook's protocol is
+    warm-up then timed calls with a device sync) -- square-root and covariance
+    form, extended and cubature -- next to its published times."""
+    import torch
+    from parsmooth.linearization import cubature, extended
+    T = 1 << 12
+    ys = make_inputs(T)
+    res = {}
+    for mname, method in (("extended", extended), ("cubature", cubature)):
+        for form, sqrt in (("sqrt", True), ("cov", False)):
+            r, init = build_runner(T, ys, torch.float64, method=method, sqrt=sqrt)
+            mean, med = time_calls(lambda: r.run(10, init, use_graph=True), steps, warmup)
+            key = f"{mname}_{form}"
+            ref = REF_T4096_MS[key]
+            res[key] = {"ours_ms": mean * 1e3, "ours_median_ms": med * 1e3, "reference_ms": ref,
+                        "speedup_vs_rtx_3080_ti": ref["rtx_3080_ti"] / (mean * 1e3),
+                        "speedup_vs_tesla": ref["tesla"] / (mean * 1e3)}
+            del r, init
+    torch.cuda.empty_cache()
+    return res
 
 
 def fp32_stability(Ts, n_runs=100, n_iter=20):
@@ -540,11 +579,13 @@ def main():
                 "errors_over": "the sequences at a fixed point (summed gradient, summed ell)",
                 "grad_fp64": g64.tolist()}
 
-    sweep = lin = errs = stab = None
+    sweep = lin = errs = stab = pub = None
     if args.sweep:
         log(f"T sweep {args.sweep}")
         sweep = t_sweep([int(x) for x in args.sweep.split(",")], n_iter, args.steps, args.warmup)
         lin = linear_cv_case(max(args.steps, 20), args.warmup)
+        log("published reference cases, T=4096")
+        pub = published_cases(max(args.steps, 10), args.warmup)
     if args.fp32_errors:
         log(f"FP32 cubature errors {args.fp32_errors}")
         errs = fp32_cubature_errors([int(x) for x in args.fp32_errors.split(",")], 5)
@@ -591,6 +632,7 @@ def main():
         "linear_cv_T1024": lin,
         "fp32_cubature_error_per_T": errs,
         "fp32_stability": stab,
+        "reference_published_T4096": pub,
         "roofline": roof,
         "cpu_baseline": cpu,
     }
