@@ -260,7 +260,8 @@ def published_cases(steps, warmup):
     """The reference's own GPU benchmark cases at T=4096 (bearings-only, 10
     iterations, FP64, nominal [-1, -1, 6, 4, 2] / I; the notebook's protocol is
     warm-up then timed calls with a device sync) -- square-root and covariance
-    form, extended and cubature -- next to its published times."""
+    form, extended and cubature -- next to its published times.  Each case runs
+    the schedule psm_autotune picks for it, as the headline does."""
     import torch
     from parsmooth.linearization import cubature, extended
     T = 1 << 12
@@ -269,10 +270,12 @@ def published_cases(steps, warmup):
     for mname, method in (("extended", extended), ("cubature", cubature)):
         for form, sqrt in (("sqrt", True), ("cov", False)):
             r, init = build_runner(T, ys, torch.float64, method=method, sqrt=sqrt)
+            tuned = r.autotune(init)
             mean, med = time_calls(lambda: r.run(10, init, use_graph=True), steps, warmup)
             key = f"{mname}_{form}"
             ref = REF_T4096_MS[key]
             res[key] = {"ours_ms": mean * 1e3, "ours_median_ms": med * 1e3, "reference_ms": ref,
+                        "schedule": _sched(tuned, r),
                         "speedup_vs_rtx_3080_ti": ref["rtx_3080_ti"] / (mean * 1e3),
                         "speedup_vs_tesla": ref["tesla"] / (mean * 1e3)}
             del r, init
