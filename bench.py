@@ -27,7 +27,7 @@ NX, NY = 5, 2
 # algorithmic figures per step-iteration (one state through one filter + smoother pass), nx=5 ny=2
 ALG_BYTES_WORDS = NY + 4 * (NX + NX * (NX + 1) // 2)     # 82 words
 ALG_FLOPS = 3793                                       # work-optimal sqrt filter + smoother
-# algorithmic work of the dominant kernels (see DESIGN.md): flops per unit processed
+# algorithmic work of the dominant kernels: flops per unit processed
 # per state (phase kernels) or per chunk aggregate (scan kernels, 2 combines per item)
 KERNEL_FLOPS = {
     "k_filter_reduce": ("state", 1600),     # one step absorbed into a chunk aggregate
@@ -245,9 +245,9 @@ REF_FP32_NAN_PCT = {
 }
 
 
-# Reference-published GPU times at T=4096 (BASELINE 1.1 / 1.2, from
-# notebooks/new_outputs/rtx-GPU-all-methods.csv and notebooks/outputs/GPU-all-methods
-# line 11): mean wall time per call of iterated_smoothing, 10 iterations, FP64.
+# Reference-published GPU times at T=4096 (notebooks/new_outputs/rtx-GPU-all-methods.csv
+# and notebooks/outputs/GPU-all-methods, line 11): mean wall time per call of
+# iterated_smoothing, 10 iterations, FP64.
 REF_T4096_MS = {
     "extended_sqrt": {"rtx_3080_ti": 301.2, "tesla": 298.3},
     "cubature_sqrt": {"rtx_3080_ti": 375.1, "tesla": 272.5},
