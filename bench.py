@@ -283,6 +283,122 @@ def published_cases(steps, warmup):
     return res
 
 
+def _ct_torch(dt):
+    """Coordinated-turn transition in torch operations (a user-supplied model)."""
+    import torch
+
+    def f(x):
+        w = x[..., 4]
+        c, s = torch.cos(w * dt), torch.sin(w * dt)
+        small = w.abs() < 1e-6
+        a = torch.where(small, torch.full_like(w, dt), s / torch.where(small, torch.ones_like(w), w))
+        bb = torch.where(small, torch.zeros_like(w), (c - 1) / torch.where(small, torch.ones_like(w), w))
+        return torch.stack([x[..., 0] + a * x[..., 2] - bb * x[..., 3], x[..., 1] + bb * x[..., 2] + a * x[..., 3],
+                            c * x[..., 2] + s * x[..., 3], -s * x[..., 2] + c * x[..., 3], w], -1)
+    return f
+
+
+def _bearings_torch(s1, s2):
+    import torch
+
+    def f(x):
+        return torch.stack([torch.atan2(x[..., 1] - s1[1], x[..., 0] - s1[0]),
+                            torch.atan2(x[..., 1] - s2[1], x[..., 0] - s2[0])], -1)
+    return f
+
+
+def widening_legs(steps, warmup):
+    """The rows beyond the smoothing path, timed through their public API:
+    the pathwise sampler, conditional-moments models, torch-callable model
+    functions (against the same model built into the kernels) and the
+    L-BFGS-B parameter-estimation driver.  FP64."""
+    import math
+    import torch
+    import parsmooth
+    from parsmooth import ConditionalMomentsModel, FunctionalModel, MVNSqrt
+    from parsmooth import _native as N
+    from parsmooth.estimation import fit
+    from parsmooth.gradient import IteratedLoglikelihood
+    from parsmooth.linearization import cubature, extended
+    from parsmooth.models import make_parameters
+    dev = torch.device("cuda", 0)
+    t = lambda a: torch.as_tensor(np.asarray(a, dtype=np.float64), device=dev)
+    res = {}
+    T = 1 << 14
+    ys = make_inputs(T)
+    Q, R, obs_f, trans_f = make_parameters(0.01, 0.1, 0.5, 0.01, (-1.5, 0.5), (1.0, 1.0))
+    cQ, cR = np.linalg.cholesky(Q), np.linalg.cholesky(R)
+    x0 = MVNSqrt(t([0.1, 0.2, 1.0, 0.0, 0.0]), t(np.eye(5)))
+    init = MVNSqrt(t(np.tile([-1.0, -1.0, 6.0, 4.0, 2.0], (T + 1, 1))), t(np.tile(np.eye(5), (T + 1, 1, 1))))
+    crit = lambda i, *_: i < 10
+
+    # built-in vs torch-callable models, extended IEKS, 10 iterations
+    builtin = (FunctionalModel(trans_f, MVNSqrt(t(np.zeros(5)), t(cQ))),
+               FunctionalModel(obs_f, MVNSqrt(t(np.zeros(2)), t(cR))))
+    torchf = (FunctionalModel(_ct_torch(0.01), MVNSqrt(t(np.zeros(5)), t(cQ))),
+              FunctionalModel(_bearings_torch((-1.5, 0.5), (1.0, 1.0)), MVNSqrt(t(np.zeros(2)), t(cR))))
+    row = {"workload": f"bearings-only IEKS, T={T}, 10 iterations, FP64, parsmooth.iterated_smoothing"}
+    for name, (tm, om) in (("builtin_ms", builtin), ("torch_callable_ms", torchf)):
+        mean, _ = time_calls(lambda: parsmooth.iterated_smoothing(t(ys), x0, tm, om, extended, init,
+                                                                  criterion=crit), steps, warmup)
+        row[name] = mean * 1e3
+    res["torch_callable_models"] = row
+
+    # pathwise sampler: 64 samples of the smoothed bearings trajectory
+    S = 64
+    nom = parsmooth.iterated_smoothing(t(ys), x0, *builtin, extended, init, criterion=lambda i, *_: i < 3)
+    filt = parsmooth.filtering(t(ys), x0, *builtin, extended, nom)
+    mean, _ = time_calls(lambda: parsmooth.sampling(0, S, builtin[0], filt, extended, nom), steps, warmup)
+    res["sampler"] = {"workload": f"bearings-only, T={T}, {S} samples, extended, FP64, parsmooth.sampling",
+                      "ms": mean * 1e3, "sampled_states_per_s": (T + 1) * S / mean}
+
+    # conditional-moments models: population model, cubature IPLS
+    rng = np.random.RandomState(3)
+    x = np.empty(T + 1)
+    x[0] = 1.0
+    for k in range(T):
+        x[k + 1] = math.log(44.7) + x[k] - math.exp(x[k]) + 0.3 * rng.randn()
+    yp = rng.poisson(10.0 * np.exp(x[1:]))[:, None].astype(np.float64)
+    tm = ConditionalMomentsModel(lambda z: math.log(44.7) + z - torch.exp(z),
+                                 lambda z: torch.full((1, 1), 0.3, dtype=z.dtype, device=z.device))
+    om = ConditionalMomentsModel(lambda z: 10.0 * torch.exp(z), lambda z: torch.sqrt(10.0 * torch.exp(z)).reshape(1, 1))
+    px0 = MVNSqrt(t([1.0]), t([[1.0]]))
+    mean, _ = time_calls(lambda: parsmooth.iterated_smoothing(t(yp), px0, tm, om, cubature,
+                                                              criterion=lambda i, *_: i < 10), steps, warmup)
+    res["conditional_moments"] = {"workload": f"population model (Ricker + Poisson), T={T}, cubature IPLS, "
+                                              "10 iterations, FP64", "ms": mean * 1e3,
+                                  "step_its_per_s": T * 10 / mean}
+
+    # parameter estimation: L-BFGS-B on (log q, log r) of a linear model
+    Fm, Hm, Tp = np.array([[1.0, 0.1], [0.0, 1.0]]), np.array([[1.0, 0.0]]), 4096
+
+    def build(theta):
+        q, r = torch.exp(theta[0]), torch.exp(theta[1])
+        return {"trans_params": torch.tensor(Fm.reshape(-1), dtype=theta.dtype),
+                "trans_q_mean": torch.zeros(2, dtype=theta.dtype),
+                "trans_q_chol": torch.sqrt(q) * torch.eye(2, dtype=theta.dtype),
+                "obs_params": torch.tensor(Hm.reshape(-1), dtype=theta.dtype),
+                "obs_q_mean": torch.zeros(1, dtype=theta.dtype), "obs_q_chol": r.reshape(1, 1),
+                "m0": torch.zeros(1, 2, dtype=theta.dtype), "L0": torch.eye(2, dtype=theta.dtype)[None]}
+    from parsmooth.data import lgssm_data
+    _, yl = lgssm_data(np.zeros(2), Fm, Hm, np.array([[0.09]]), 0.05 * np.eye(2), np.zeros(2), np.zeros(1), Tp,
+                       random_state=np.random.RandomState(0))
+    obj = IteratedLoglikelihood(build, N.MODEL_LINEAR, N.MODEL_LINEAR, 2, 1, [Tp], np.asarray(yl, np.float64),
+                                extended, 2, (np.zeros((Tp + 1, 2)), np.tile(np.eye(2), (Tp + 1, 1, 1))))
+    fit(obj, np.array([0.0, 0.0]), bounds=[(-8.0, 3.0), (-8.0, 3.0)])   # warm-up
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    out = fit(obj, np.array([0.0, 0.0]), bounds=[(-8.0, 3.0), (-8.0, 3.0)])
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    res["parameter_estimation"] = {"workload": f"linear model nx=2 ny=1, T={Tp}, L-BFGS-B on (log q, log r), "
+                                               "value and gradient on the device, FP64",
+                                   "ms_per_fit": dt * 1e3, "evaluations": int(out.nfev),
+                                   "converged": bool(out.success)}
+    torch.cuda.empty_cache()
+    return res
+
+
 def fp32_stability(Ts, n_runs=100, n_iter=20):
     """The reference's FP32 robustness protocol (notebooks/robustness_100runs.py:
     100 bearings-only runs, each a prefix of length T of one T=8000 simulation;
@@ -582,13 +698,15 @@ def main():
                 "errors_over": "the sequences at a fixed point (summed gradient, summed ell)",
                 "grad_fp64": g64.tolist()}
 
-    sweep = lin = errs = stab = pub = None
+    sweep = lin = errs = stab = pub = wide = None
     if args.sweep:
         log(f"T sweep {args.sweep}")
         sweep = t_sweep([int(x) for x in args.sweep.split(",")], n_iter, args.steps, args.warmup)
         lin = linear_cv_case(max(args.steps, 20), args.warmup)
         log("published reference cases, T=4096")
         pub = published_cases(max(args.steps, 10), args.warmup)
+        log("sampler, conditional moments, torch-callable models, parameter estimation")
+        wide = widening_legs(args.steps, 1)
     if args.fp32_errors:
         log(f"FP32 cubature errors {args.fp32_errors}")
         errs = fp32_cubature_errors([int(x) for x in args.fp32_errors.split(",")], 5)
@@ -636,6 +754,7 @@ def main():
         "fp32_cubature_error_per_T": errs,
         "fp32_stability": stab,
         "reference_published_T4096": pub,
+        "beyond_the_smoothing_path": wide,
         "roofline": roof,
         "cpu_baseline": cpu,
     }

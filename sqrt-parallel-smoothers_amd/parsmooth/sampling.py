@@ -29,7 +29,8 @@ def _normals(key, shape, dtype, dev) -> torch.Tensor:
             raise ValueError(f"normals must have shape {tuple(shape)}, got {tuple(eps.shape)}")
         return eps.contiguous()
     if isinstance(key, torch.Generator):
-        return torch.randn(shape, generator=key, device=dev, dtype=dtype)
+        # draw on the generator's own device (a CPU generator is accepted) and move
+        return torch.randn(shape, generator=key, device=key.device, dtype=dtype).to(dev)
     gen = torch.Generator(device=dev)
     gen.manual_seed(int(key))
     return torch.randn(shape, generator=gen, device=dev, dtype=dtype)

@@ -96,3 +96,27 @@ def test_sampling_moments_and_default_nominal():
     d = xs - xs.mean(1, keepdims=True)
     C = np.einsum("tsi,tsj->tij", d, d) / (S - 1)
     np.testing.assert_allclose(C, P, rtol=0.05, atol=0.05 * np.abs(P).max())
+
+
+def test_sampling_key_forms_agree():
+    """The key may be normals, a seed or a torch.Generator on any device: a CPU
+    generator draws on the CPU and gives exactly the samples of the same normals
+    passed as a tensor."""
+    np.random.seed(9)
+    x0, cx0, F, Q, cholQ, b, _ = O.get_system(2, 2)
+    _, _, H, R, cholR, c, _ = O.get_system(2, 1)
+    F = 0.9 * F / max(1.0, np.abs(np.linalg.eigvals(F)).max())
+    T, S = 50, 7
+    ys = np.random.RandomState(5).randn(T, 1)
+    tq = O.FunctionalModel(O.Linear(F), O.MVNSqrt(b, cholQ))
+    to = O.FunctionalModel(O.Linear(H), O.MVNSqrt(c, cholR))
+    filt = O.filtering(ys, cx0, tq, to, O.extended, None, True)
+    td = FunctionalModel(M.LinearFunction(F), MVNSqrt(t64(b), t64(cholQ)))
+    ft = MVNSqrt(t64(filt.mean), t64(filt.chol))
+    eps = torch.randn((T + 1, S, 2), generator=torch.Generator().manual_seed(11), dtype=torch.float64)
+    a = npy(parsmooth.sampling(torch.Generator().manual_seed(11), S, td, ft, extended))
+    b_ = npy(parsmooth.sampling(eps.to(DEV), S, td, ft, extended))
+    np.testing.assert_array_equal(a, b_)
+    s1 = npy(parsmooth.sampling(77, S, td, ft, extended))
+    s2 = npy(parsmooth.sampling(77, S, td, ft, extended))
+    np.testing.assert_array_equal(s1, s2)
