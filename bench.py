@@ -570,7 +570,7 @@ def main():
     ap.add_argument("--n-iter", type=int, default=10)
     ap.add_argument("--graph", type=int, default=1)
     ap.add_argument("--cpu-sample", type=int, default=131072)
-    ap.add_argument("--cpu-repeats", type=int, default=4)
+    ap.add_argument("--cpu-repeats", type=int, default=10)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--batch-seqs", type=int, default=4096)
     ap.add_argument("--grad-seqs", type=int, default=256)

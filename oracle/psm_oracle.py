@@ -29,9 +29,13 @@ Parity status (see tests/test_oracle_*.py for the pins):
   sqrt==std and par==seq relations, reference fixtures).
 * fixed_point_vjp (implicit-gradient restatement): pinned by the scalar known
   answer of tests/test_math_utils.py:57-67 and by finite differences.
+* get_conditional_model: pinned by the reference's linear-model test
+  (tests/test_linearization.py:238-283), by models affine in q (exact
+  conditional moments for every method) and by a log-normal closed form.
 """
 from __future__ import annotations
 
+import functools
 import math
 from typing import Callable, NamedTuple, Optional
 
@@ -459,6 +463,98 @@ def linearize_many(method, model, traj):
     Both oracle linearizations broadcast over a leading axis, so this is a
     single batched call."""
     return method(model, type(traj)(np.asarray(traj[0], np.float64), np.asarray(traj[1], np.float64)))
+
+
+# ----------------------------------------------------------------------------
+# doubly non-linear models (parsmooth/linearization/_common.py:17-66)
+# ----------------------------------------------------------------------------
+def method_table(method, n):
+    """The sigma table ``method`` linearizes with at dimension n (None for the
+    extended method); a functools.partial passes its keywords on."""
+    kw = {}
+    while isinstance(method, functools.partial):
+        kw = {**method.keywords, **kw}
+        method = method.func
+    if method is extended:
+        return None
+    if method is cubature:
+        return cubature_table(n)
+    if method is gauss_hermite:
+        return gauss_hermite_table(n, **kw)
+    if method is unscented:
+        return unscented_table(n, **kw)
+    raise TypeError(f"unknown linearization method {method!r}")
+
+
+class _AtFixedX:
+    """q -> f(x, q) at one x: the function of the inner FunctionalModel
+    (parsmooth/linearization/_common.py:53,59)."""
+
+    def __init__(self, f, x):
+        self.f, self.x = f, x
+
+    def __call__(self, q):
+        return self.f(self.x, q)
+
+    def jac(self, q):
+        return self.f.jac_q(self.x, q)
+
+
+class _NestedMoment:
+    """The conditional mean (``which`` 'mean', :52-56) or chol / covariance
+    (:58-64) of f(x, q) given x, by linearizing q -> f(x, q) under q with
+    ``method``.  Callable on (..., n) points, one inner linearization per point."""
+
+    def __init__(self, f, q, method, which):
+        self.f, self.q, self.method, self.which = f, q, method, which
+        self.sqrt = isinstance(q, MVNSqrt)
+        zm = np.zeros_like(np.asarray(q.mean, np.float64))
+        zc = np.zeros_like(np.asarray(q[1], np.float64))
+        self.noise = MVNSqrt(zm, zc) if self.sqrt else MVNStandard(zm, zc)
+
+    def _one(self, x):
+        F, val, bias = self.method(FunctionalModel(_AtFixedX(self.f, x), self.noise), self.q)
+        if self.which == "mean":
+            return F @ np.asarray(self.q.mean, np.float64) + bias
+        if self.sqrt:
+            return tria(np.concatenate([F @ np.asarray(self.q.chol, np.float64), val], axis=1))
+        return F @ np.asarray(self.q.cov, np.float64) @ F.T + val
+
+    def __call__(self, x):
+        x = np.asarray(x, np.float64)
+        outs = [self._one(p) for p in x.reshape(-1, x.shape[-1])]
+        out = np.stack(outs)
+        return out.reshape(x.shape[:-1] + out.shape[1:])
+
+    def jac(self, x):
+        """Jacobian of the conditional mean in x.  F m_q + b with
+        b = (weighted mean of f(x, q_j)) - F m_q + 0 is that weighted mean, so the
+        Jacobian is the same weighted mean of df/dx(x, q_j) (f(x, m_q) alone for
+        the extended method)."""
+        x = np.asarray(x, np.float64)
+        qs = get_mvnsqrt(self.q)
+        m, L = np.asarray(qs.mean, np.float64), np.asarray(qs.chol, np.float64)
+        table = method_table(self.method, m.shape[-1])
+        if table is None:
+            pts, w = m[None], np.ones(1)
+        else:
+            pts, w = m[None] + table.xi @ L.T, table.wm
+        outs = []
+        for p in x.reshape(-1, x.shape[-1]):
+            outs.append(sum(wj * self.f.jac_x(p, qj) for wj, qj in zip(w, pts)))
+        out = np.stack(outs)
+        return out.reshape(x.shape[:-1] + out.shape[1:])
+
+
+def get_conditional_model(f, q, linearization_method):
+    """parsmooth/linearization/_common.py:17-66.  ``f`` is callable as f(x, q)
+    (q may carry leading batch axes) with ``jac_x`` and ``jac_q`` methods."""
+    try:
+        f(np.asarray(q.mean, np.float64), np.asarray(q.mean, np.float64))
+    except Exception:  # noqa: BLE001 - the reference catches everything here (:43-46)
+        raise NotImplementedError("`x` and `q` with different dimensions are not supported yet.")
+    return ConditionalMomentsModel(_NestedMoment(f, q, linearization_method, "mean"),
+                                   _NestedMoment(f, q, linearization_method, "chol_or_cov"))
 
 
 # ----------------------------------------------------------------------------

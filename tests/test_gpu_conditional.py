@@ -88,3 +88,102 @@ def test_population_model_filter_smoother_matches_oracle(method, sqrt):
     # and the iterated smoother with the default nominal runs end to end
     res = parsmooth.iterated_smoothing(t64(ys), x0d, tm, om, METHODS[method], criterion=lambda i, *_: i < 5)
     assert torch.isfinite(res[0]).all()
+
+
+# ------------------------------------------- get_conditional_model on the device
+from parsmooth import FunctionalModel  # noqa: E402
+from parsmooth.linearization import get_conditional_model  # noqa: E402
+from parsmooth.models import LinearFunction  # noqa: E402
+
+from test_conditional_model import lin_xq_torch, mult_noise_torch  # noqa: E402
+from test_oracle import _LinXQ, _MultNoise, _rand_lower  # noqa: E402
+
+
+@pytest.mark.parametrize("dim", [1, 2])
+@pytest.mark.parametrize("seed", [0, 42])
+@pytest.mark.parametrize("method", list(METHODS))
+@pytest.mark.parametrize("sqrt", [True, False])
+def test_get_conditional_model_linear_on_device(dim, seed, method, sqrt):
+    """tests/test_linearization.py:238-283 through psm_linearize: the outer
+    linearization of the conditional model of a x + b q + c recovers a, the
+    mean and (b chol_q)(b chol_q)^T, and equals the oracle's."""
+    np.random.seed(seed)
+    a, b, c = np.random.randn(dim, dim), np.random.randn(dim, dim), np.random.randn(dim)
+    m_x, m_q = np.random.randn(dim), np.random.randn(dim)
+    chol_x, chol_q = _rand_lower(dim), _rand_lower(dim)
+    if sqrt:
+        x_d, q_d = MVNSqrt(t64(m_x), t64(chol_x)), MVNSqrt(t64(m_q), t64(chol_q))
+        x_o, q_o = O.MVNSqrt(m_x, chol_x), O.MVNSqrt(m_q, chol_q)
+    else:
+        P, Pq = chol_x @ chol_x.T, chol_q @ chol_q.T
+        x_d, q_d = MVNStandard(t64(m_x), t64(P)), MVNStandard(t64(m_q), t64(Pq))
+        x_o, q_o = O.MVNStandard(m_x, P), O.MVNStandard(m_q, Pq)
+    model = get_conditional_model(lin_xq_torch(a, b, c), q_d, METHODS[method])
+    F, C, rem = (npy(v) for v in METHODS[method](model, x_d))
+    ref = OMETH[method](O.get_conditional_model(_LinXQ(a, b, c), q_o, OMETH[method]), x_o)
+    for u, v, name in zip((F, C, rem), ref, "FCb"):
+        close(u, v, 1e-10, f"{method} {name} vs oracle")
+    Q = C @ C.T if sqrt else C
+    xp = np.random.randn(dim)
+    close(F, a, 1e-9, "F = a")
+    close(F @ xp + rem, a @ xp + b @ m_q + c, 1e-9, "mean")
+    close(Q, (b @ chol_q) @ (b @ chol_q).T, 1e-9, "Q")
+
+
+def _mult_noise_problem(T, seed, sqrt, method):
+    """2-D state through f(x, q) = g(x) exp(0.2 q) + 0.1 q, q ~ N(0, 0.3^2 I),
+    observed as x0 + 0.5 x1 with noise 0.2."""
+    rng = np.random.RandomState(seed)
+    f = _MultNoise()
+    sq = 0.3
+    x = np.empty((T + 1, 2))
+    x[0] = [0.5, -0.2]
+    for k in range(T):
+        x[k + 1] = f(x[k], sq * rng.randn(2))
+    H = np.array([[1.0, 0.5]])
+    ys = x[1:] @ H.T + 0.2 * rng.randn(T, 1)
+    m0, L0 = np.array([0.5, -0.2]), 0.5 * np.eye(2)
+    Lq, Lr = sq * np.eye(2), np.array([[0.2]])
+    if sqrt:
+        q_d, q_o = MVNSqrt(t64(np.zeros(2)), t64(Lq)), O.MVNSqrt(np.zeros(2), Lq)
+        obs_d = FunctionalModel(LinearFunction(H), MVNSqrt(t64(np.zeros(1)), t64(Lr)))
+        obs_o = O.FunctionalModel(O.Linear(H), O.MVNSqrt(np.zeros(1), Lr))
+        x0d, x0o = MVNSqrt(t64(m0), t64(L0)), O.MVNSqrt(m0, L0)
+    else:
+        q_d, q_o = MVNStandard(t64(np.zeros(2)), t64(Lq @ Lq.T)), O.MVNStandard(np.zeros(2), Lq @ Lq.T)
+        obs_d = FunctionalModel(LinearFunction(H), MVNStandard(t64(np.zeros(1)), t64(Lr @ Lr.T)))
+        obs_o = O.FunctionalModel(O.Linear(H), O.MVNStandard(np.zeros(1), Lr @ Lr.T))
+        x0d, x0o = MVNStandard(t64(m0), t64(L0 @ L0.T)), O.MVNStandard(m0, L0 @ L0.T)
+    trans_d = get_conditional_model(mult_noise_torch, q_d, METHODS[method])
+    trans_o = O.get_conditional_model(_MultNoise(), q_o, OMETH[method])
+    return ys, x, (x0d, trans_d, obs_d), (x0o, trans_o, obs_o), rng
+
+
+@pytest.mark.parametrize("method", list(METHODS))
+@pytest.mark.parametrize("sqrt", [True, False])
+def test_noise_nonlinear_filter_smoother_matches_oracle(method, sqrt):
+    """A transition non-linear in its noise, through get_conditional_model:
+    the filter (with log-likelihood) and smoother at a fixed nominal trajectory
+    and three iterations of the iterated smoother match the oracle to 1e-9."""
+    T = 60
+    ys, x, (x0d, td, od), (x0o, to, oo), rng = _mult_noise_problem(T, 5, sqrt, method)
+    nm = x + 0.05 * rng.randn(T + 1, 2)
+    nL = np.tile(0.3 * np.eye(2), (T + 1, 1, 1))
+    if sqrt:
+        nomd, nomo = MVNSqrt(t64(nm), t64(nL)), O.MVNSqrt(nm, nL)
+    else:
+        nomd, nomo = MVNStandard(t64(nm), t64(nL @ nL)), O.MVNStandard(nm, nL @ nL)
+    filt, ell = parsmooth.filtering(t64(ys), x0d, td, od, METHODS[method], nomd, return_loglikelihood=True)
+    rf, rell = O.filtering(ys, x0o, to, oo, OMETH[method], nomo, return_loglikelihood=True)
+    close(npy(filt[0]), rf[0], 1e-9, "filtered mean")
+    close(npy(filt[1]), rf[1], 1e-9, "filtered chol/cov")
+    close(float(ell), rell, 1e-9, "log-likelihood")
+    got = parsmooth.filter_smoother(t64(ys), x0d, td, od, METHODS[method], nomd)
+    ref = O.filter_smoother(ys, x0o, to, oo, OMETH[method], nomo, True)
+    close(npy(got[0]), ref[0], 1e-9, "smoothed mean")
+    close(npy(got[1]), ref[1], 1e-9, "smoothed chol/cov")
+    crit = lambda i, *_: i < 3
+    got = parsmooth.iterated_smoothing(t64(ys), x0d, td, od, METHODS[method], nomd, criterion=crit)
+    ref = O.iterated_smoothing(ys, x0o, to, oo, OMETH[method], nomo, criterion=crit)
+    close(npy(got[0]), ref[0], 1e-9, "iterated mean")
+    close(npy(got[1]), ref[1], 1e-9, "iterated chol/cov")

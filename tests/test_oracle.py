@@ -521,3 +521,156 @@ def test_conditional_sqrt_vs_std(seed, method):
         np.testing.assert_allclose(Fs, F, atol=1e-10)
         np.testing.assert_allclose(bs, b, atol=1e-10)
         np.testing.assert_allclose(Ls @ Ls.T, P, atol=1e-10)
+
+
+# ------------------------------------------------- models non-linear in the noise
+class _LinXQ:
+    """f(x, q) = a x + b q + c (tests/test_linearization.py:243-252)."""
+
+    def __init__(self, a, b, c):
+        self.a, self.b, self.c = a, b, c
+
+    def __call__(self, x, q):
+        # matmul, not einsum: a dimension mismatch must raise, as a @ x does in the reference
+        return np.asarray(x) @ self.a.T + np.asarray(q) @ self.b.T + self.c
+
+    def jac_x(self, x, q):
+        return self.a.copy()
+
+    def jac_q(self, x, q):
+        return self.b.copy()
+
+
+class _AffineInQ:
+    """f(x, q) = g(x) + B(x) q on R^2: g(x) = (sin x0 + x1, 0.5 x0 x1),
+    B(x) = [[1 + 0.1 x1^2, 0.2], [0.3 x0, 1]].  Every method reproduces the
+    conditional moments g(x) + B(x) m_q and B(x) P_q B(x)^T exactly."""
+
+    @staticmethod
+    def g(x):
+        return np.array([np.sin(x[0]) + x[1], 0.5 * x[0] * x[1]])
+
+    @staticmethod
+    def B(x):
+        return np.array([[1.0 + 0.1 * x[1] ** 2, 0.2], [0.3 * x[0], 1.0]])
+
+    def __call__(self, x, q):
+        return self.g(x) + np.einsum("ij,...j->...i", self.B(x), q)
+
+    def jac_x(self, x, q):
+        Jg = np.array([[np.cos(x[0]), 1.0], [0.5 * x[1], 0.5 * x[0]]])
+        dB = np.array([[0.0, 0.2 * x[1] * q[0]], [0.3 * q[0], 0.0]])   # d(B(x) q)/dx
+        return Jg + dB
+
+    def jac_q(self, x, q):
+        return self.B(x)
+
+
+class _MultNoise:
+    """f(x, q) = g(x) * exp(s q) + r q on R^2 with g(x) = (x0 + 0.1 x1,
+    0.95 x1 + 0.1 sin x0), s = 0.2, r = 0.1: noise that is multiplicative and
+    additive at once."""
+    s, r = 0.2, 0.1
+
+    @staticmethod
+    def g(x):
+        return np.array([x[0] + 0.1 * x[1], 0.95 * x[1] + 0.1 * np.sin(x[0])])
+
+    def __call__(self, x, q):
+        return self.g(x) * np.exp(self.s * np.asarray(q)) + self.r * np.asarray(q)
+
+    def jac_x(self, x, q):
+        Jg = np.array([[1.0, 0.1], [0.1 * np.cos(x[0]), 0.95]])
+        return np.exp(self.s * np.asarray(q))[:, None] * Jg
+
+    def jac_q(self, x, q):
+        return np.diag(self.s * self.g(x) * np.exp(self.s * np.asarray(q)) + self.r)
+
+
+class _LogNormal:
+    """f(x, q) = x exp(q) in one dimension: E[f | x] = x exp(m + s^2 / 2),
+    Var[f | x] = x^2 (exp(s^2) - 1) exp(2 m + s^2) for q ~ N(m, s^2)."""
+
+    def __call__(self, x, q):
+        return np.asarray(x) * np.exp(np.asarray(q))
+
+    def jac_x(self, x, q):
+        return np.exp(np.asarray(q)).reshape(1, 1)
+
+    def jac_q(self, x, q):
+        return (np.asarray(x) * np.exp(np.asarray(q))).reshape(1, 1)
+
+
+@pytest.mark.parametrize("dim_x", [1, 2])
+@pytest.mark.parametrize("dim_q", [1, 2])
+@pytest.mark.parametrize("seed", [0, 42])
+@pytest.mark.parametrize("method", list(METHODS))
+@pytest.mark.parametrize("sqrt", [True, False])
+def test_get_conditional_model_linear(dim_x, dim_q, seed, method, sqrt):
+    """tests/test_linearization.py:238-283: the outer linearization of the
+    conditional model of a x + b q + c recovers a, the mean and (b chol_q)(b chol_q)^T."""
+    np.random.seed(seed)
+    a, b, c = np.random.randn(dim_x, dim_x), np.random.randn(dim_x, dim_q), np.random.randn(dim_x)
+    m_x, m_q = np.random.randn(dim_x), np.random.randn(dim_q)
+    chol_x, chol_q = _rand_lower(dim_x), _rand_lower(dim_q)
+    if sqrt:
+        x_mvn, q_mvn = O.MVNSqrt(m_x, chol_x), O.MVNSqrt(m_q, chol_q)
+    else:
+        x_mvn, q_mvn = O.MVNStandard(m_x, chol_x @ chol_x.T), O.MVNStandard(m_q, chol_q @ chol_q.T)
+    f = _LinXQ(a, b, c)
+    if dim_x != dim_q:
+        with pytest.raises(NotImplementedError):
+            O.get_conditional_model(f, q_mvn, METHODS[method])
+        return
+    model = O.get_conditional_model(f, q_mvn, METHODS[method])
+    F_x, Q_lin, rem = METHODS[method](model, x_mvn)
+    if sqrt:
+        Q_lin = Q_lin @ Q_lin.T
+    x_prime = np.random.randn(dim_x)
+    np.testing.assert_allclose(F_x, a, atol=1e-9)
+    np.testing.assert_allclose(F_x @ x_prime + rem, a @ x_prime + b @ m_q + c, atol=1e-9)
+    np.testing.assert_allclose(Q_lin, (b @ chol_q) @ (b @ chol_q).T, atol=1e-9)
+
+
+@pytest.mark.parametrize("method", list(METHODS))
+@pytest.mark.parametrize("sqrt", [True, False])
+def test_get_conditional_model_affine_in_q(method, sqrt):
+    """Non-linear in x, affine in q: the conditional moments are exact for every
+    inner method, and the conditional mean's Jacobian is dg/dx + d(B(x) m_q)/dx."""
+    rng = np.random.RandomState(7)
+    m_q = rng.randn(2)
+    cq = _rand_lower(2, rng) + np.eye(2) * 0.2
+    q = O.MVNSqrt(m_q, cq) if sqrt else O.MVNStandard(m_q, cq @ cq.T)
+    f = _AffineInQ()
+    model = O.get_conditional_model(f, q, METHODS[method])
+    xs = rng.randn(6, 2)
+    got_m = model.conditional_mean(xs)
+    got_c = model.conditional_covariance_or_cholesky(xs)
+    got_J = model.conditional_mean.jac(xs)
+    for k, x in enumerate(xs):
+        B = f.B(x)
+        np.testing.assert_allclose(got_m[k], f.g(x) + B @ m_q, rtol=1e-12, atol=1e-12)
+        cov = got_c[k] @ got_c[k].T if sqrt else got_c[k]
+        np.testing.assert_allclose(cov, B @ cq @ cq.T @ B.T, rtol=1e-11, atol=1e-12)
+        if sqrt:
+            assert np.allclose(got_c[k], np.tril(got_c[k])) and np.all(np.diag(got_c[k]) >= 0)
+        np.testing.assert_allclose(got_J[k], f.jac_x(x, m_q), rtol=1e-12, atol=1e-12)
+
+
+@pytest.mark.parametrize("sqrt", [True, False])
+def test_get_conditional_model_lognormal(sqrt):
+    """x exp(q) under Gauss-Hermite with 12 nodes: the log-normal mean and
+    variance to 1e-11 (the variance is mostly the non-linear residual
+    Phi - F P F^T, so a dropped residual fails here)."""
+    import functools
+    m, s = 0.3, 0.4
+    q = O.MVNSqrt(np.array([m]), np.array([[s]])) if sqrt else O.MVNStandard(np.array([m]), np.array([[s * s]]))
+    model = O.get_conditional_model(_LogNormal(), q, functools.partial(O.gauss_hermite, order=12))
+    xs = np.array([[0.5], [-1.3], [2.0]])
+    mean = model.conditional_mean(xs)[:, 0]
+    fac = model.conditional_covariance_or_cholesky(xs)[:, 0, 0]
+    var = fac * fac if sqrt else fac
+    x = xs[:, 0]
+    np.testing.assert_allclose(mean, x * np.exp(m + s * s / 2), rtol=1e-11)
+    np.testing.assert_allclose(var, x * x * np.expm1(s * s) * np.exp(2 * m + s * s), rtol=1e-10)
+    np.testing.assert_allclose(model.conditional_mean.jac(xs)[:, 0, 0], np.exp(m + s * s / 2), rtol=1e-11)
