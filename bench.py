@@ -310,8 +310,8 @@ def _bearings_torch(s1, s2):
 def widening_legs(steps, warmup):
     """The rows beyond the smoothing path, timed through their public API:
     the pathwise sampler, conditional-moments models, torch-callable model
-    functions (against the same model built into the kernels) and the
-    L-BFGS-B parameter-estimation driver.  FP64."""
+    functions (against the same model built into the kernels), the sequential
+    passes (parallel=False) and the L-BFGS-B parameter-estimation driver.  FP64."""
     import math
     import torch
     import parsmooth
@@ -351,6 +351,39 @@ def widening_legs(steps, warmup):
     mean, _ = time_calls(lambda: parsmooth.sampling(0, S, builtin[0], filt, extended, nom), steps, warmup)
     res["sampler"] = {"workload": f"bearings-only, T={T}, {S} samples, extended, FP64, parsmooth.sampling",
                       "ms": mean * 1e3, "sampled_states_per_s": (T + 1) * S / mean}
+
+    # sequential passes (parallel=False, one lane per sequence; without a
+    # nominal the extended Kalman filter + RTS smoother) against the parallel
+    # filter-smoother pass: one long sequence, and a batch of short ones
+    from parsmooth._engine import Models, Problem, filter_smoother_pass, seq_filter_pass, seq_smoother_pass
+    from parsmooth.linearization._methods import resolve
+    row = {"workload": f"bearings-only filter + smoother, extended, FP64: one sequence of T={T} "
+                       "(parsmooth.filter_smoother), and 16384 sequences of T=64 in one batch"}
+    mean, _ = time_calls(lambda: parsmooth.filter_smoother(t(ys), x0, *builtin, extended, None, parallel=False),
+                         steps, warmup)
+    row["one_sequence_sequential_ms"] = mean * 1e3
+    mean, _ = time_calls(lambda: parsmooth.filter_smoother(t(ys), x0, *builtin, extended, init), steps, warmup)
+    row["one_sequence_parallel_ms"] = mean * 1e3
+    nb, Tb = 16384, 64
+    prob = Problem(5, 2, [Tb] * nb, torch.float64, True, dev)
+    mods = Models(prob, *builtin, resolve(extended))
+    yb = t(np.tile(ys[:Tb], (nb, 1)))
+    m0b = t(np.tile([0.1, 0.2, 1.0, 0.0, 0.0], (nb, 1)))
+    L0b = t(np.tile(np.eye(5), (nb, 1, 1)))
+    nomb = prob.default_nominal()
+    fb_, ob_ = prob.new_traj(), prob.new_traj()
+
+    def seq_batch():
+        seq_filter_pass(prob, mods, yb, m0b, L0b, None, None, False, out=fb_)
+        seq_smoother_pass(prob, mods, fb_[0], fb_[1], None, None, out=ob_)
+
+    mean, _ = time_calls(seq_batch, steps, warmup)
+    row["batch_sequential_steps_per_s"] = nb * Tb / mean
+    mean, _ = time_calls(lambda: filter_smoother_pass(prob, mods, yb, m0b, L0b, nomb[0], nomb[1], fb_, ob_),
+                         steps, warmup)
+    row["batch_parallel_steps_per_s"] = nb * Tb / mean
+    res["sequential_passes"] = row
+    del prob, mods, yb, nomb, fb_, ob_
 
     # conditional-moments models: population model, cubature IPLS
     rng = np.random.RandomState(3)
@@ -705,7 +738,7 @@ def main():
         lin = linear_cv_case(max(args.steps, 20), args.warmup)
         log("published reference cases, T=4096")
         pub = published_cases(max(args.steps, 10), args.warmup)
-        log("sampler, conditional moments, torch-callable models, parameter estimation")
+        log("sampler, conditional moments, torch-callable models, sequential passes, parameter estimation")
         wide = widening_legs(args.steps, 1)
     if args.fp32_errors:
         log(f"FP32 cubature errors {args.fp32_errors}")

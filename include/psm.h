@@ -211,6 +211,26 @@ int psm_smoother(const psm_problem_t* prob, const psm_model_t* trans, const psm_
                  const void* filt_m, const void* filt_L, const void* nom_m, const void* nom_L,
                  void* out_m, void* out_L, void* workspace, size_t workspace_bytes, void* stream);
 
+/* Sequential filter (parallel=False): one lane per sequence runs predict /
+ * update through its steps.  Replaces parsmooth/sequential/_filtering.py:11-53
+ * (:56-108 for the two forms' predict and update; log-likelihood summed per
+ * sequence in FP64 into ell, which may be NULL).  nom_m == NULL: the
+ * transition is linearized at the filtered state and the observation at the
+ * predicted state (:36-41); else both at the nominal trajectory.  Models of
+ * kind PSM_MODEL_TABULATED / PSM_MODEL_CONDITIONAL need a nominal.  PSM_F32
+ * and PSM_F64 only; no workspace. */
+int psm_seq_filter(const psm_problem_t* prob, const psm_model_t* trans, const psm_model_t* obs,
+                   const psm_sigma_t* sigma, const void* ys, const void* m0, const void* L0,
+                   const void* nom_m, const void* nom_L, void* out_m, void* out_L, double* ell,
+                   void* stream);
+
+/* Sequential (Rauch-Tung-Striebel) smoother, one lane per sequence.  Replaces
+ * parsmooth/sequential/_smoothing.py:12-77.  nom_m == NULL: the transition out
+ * of state k is linearized at the smoothed state k+1 (:27-33). */
+int psm_seq_smoother(const psm_problem_t* prob, const psm_model_t* trans, const psm_sigma_t* sigma,
+                     const void* filt_m, const void* filt_L, const void* nom_m, const void* nom_L,
+                     void* out_m, void* out_L, void* stream);
+
 /* One iteration of the fixed-point map: filter then smoother at the same
  * nominal (parsmooth/methods.py:38-47), sharing the transition linearization.
  * filt_m/filt_L receive the filtered trajectory.  use_graph != 0 captures the

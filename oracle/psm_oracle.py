@@ -29,6 +29,9 @@ Parity status (see tests/test_oracle_*.py for the pins):
   sqrt==std and par==seq relations, reference fixtures).
 * fixed_point_vjp (implicit-gradient restatement): pinned by the scalar known
   answer of tests/test_math_utils.py:57-67 and by finite differences.
+* seq_filtering / seq_smoothing on non-linear models: pinned by sequential ==
+  parallel at a nominal trajectory and, without one, by the fixed points of
+  their own linearization points (filtered, predicted, smoothed successor).
 * get_conditional_model: pinned by the reference's linear-model test
   (tests/test_linearization.py:238-283), by models affine in q (exact
   conditional moments for every method) and by a log-normal closed form.

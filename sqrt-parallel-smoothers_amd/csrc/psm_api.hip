@@ -393,6 +393,65 @@ int psm_sample(const psm_problem_t* p, const psm_model_t* trans, const psm_sigma
                    (hipStream_t)stream);
 }
 
+// ---------------------------------------------------------------- sequential passes
+static bool caller_evaluated(const psm_model_t* m) {
+  return m->kind == PSM_MODEL_TABULATED || m->kind == PSM_MODEL_CONDITIONAL;
+}
+
+static int check_seq(const psm_problem_t* p, const void* nom_m, const void* nom_L,
+                     const psm_model_t* a, const psm_model_t* b, const char* who) {
+  if (p->dtype != PSM_F32 && p->dtype != PSM_F64) {
+    set_error(std::string(who) + ": dtype must be PSM_F32 or PSM_F64");
+    return PSM_E_ARG;
+  }
+  if ((nom_m == nullptr) != (nom_L == nullptr)) {
+    set_error(std::string(who) + ": nom_m and nom_L must both be given or both be NULL");
+    return PSM_E_ARG;
+  }
+  if (nom_m == nullptr && (caller_evaluated(a) || (b && caller_evaluated(b)))) {
+    set_error(std::string(who) +
+              ": models evaluated by the caller need a nominal trajectory (the running "
+              "estimate is only known inside the recursion)");
+    return PSM_E_ARG;
+  }
+  return PSM_OK;
+}
+
+int psm_seq_filter(const psm_problem_t* p, const psm_model_t* trans, const psm_model_t* obs,
+                   const psm_sigma_t* sigma, const void* ys, const void* m0, const void* L0,
+                   const void* nom_m, const void* nom_L, void* out_m, void* out_L, double* ell,
+                   void* stream) {
+  int rc = check_problem(p);
+  if (rc) return rc;
+  if ((rc = check_model(trans, p->nx, p->nx))) return rc;
+  if ((rc = check_model(obs, p->nx, p->ny))) return rc;
+  if ((rc = check_seq(p, nom_m, nom_L, trans, obs, "psm_seq_filter"))) return rc;
+  if (!ys || !m0 || !L0 || !out_m || !out_L) {
+    set_error("psm_seq_filter: null array");
+    return PSM_E_ARG;
+  }
+  const OpsTable* t = need(p->dtype, p->form, p->nx, p->ny);
+  if (!t) return PSM_E_UNSUPPORTED;
+  return t->seq_filter(p, trans, obs, sigma, ys, m0, L0, nom_m, nom_L, out_m, out_L, ell,
+                       (hipStream_t)stream);
+}
+
+int psm_seq_smoother(const psm_problem_t* p, const psm_model_t* trans, const psm_sigma_t* sigma,
+                     const void* fm, const void* fL, const void* nom_m, const void* nom_L,
+                     void* out_m, void* out_L, void* stream) {
+  int rc = check_problem(p);
+  if (rc) return rc;
+  if ((rc = check_model(trans, p->nx, p->nx))) return rc;
+  if ((rc = check_seq(p, nom_m, nom_L, trans, nullptr, "psm_seq_smoother"))) return rc;
+  if (!fm || !fL || !out_m || !out_L) {
+    set_error("psm_seq_smoother: null array");
+    return PSM_E_ARG;
+  }
+  const OpsTable* t = need(p->dtype, p->form, p->nx, p->ny);
+  if (!t) return PSM_E_UNSUPPORTED;
+  return t->seq_smoother(p, trans, sigma, fm, fL, nom_m, nom_L, out_m, out_L, (hipStream_t)stream);
+}
+
 int psm_mean_sq_diff(int32_t dtype, int64_t n, const void* a, const void* b, double* out,
                      void* stream) {
   hipStream_t st = (hipStream_t)stream;

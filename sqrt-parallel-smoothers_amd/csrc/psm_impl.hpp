@@ -9,6 +9,7 @@
 
 #include "psm_kernels.hpp"
 #include "psm_sample.hpp"
+#include "psm_seq.hpp"
 
 namespace psm {
 
@@ -191,6 +192,11 @@ struct OpsTable {
   int (*sample)(const psm_problem_t*, const psm_model_t*, const psm_sigma_t*, const void*,
                 const void*, const void*, const void*, const void*, int, void*, void*, size_t,
                 hipStream_t);
+  int (*seq_filter)(const psm_problem_t*, const psm_model_t*, const psm_model_t*,
+                    const psm_sigma_t*, const void*, const void*, const void*, const void*,
+                    const void*, void*, void*, double*, hipStream_t);
+  int (*seq_smoother)(const psm_problem_t*, const psm_model_t*, const psm_sigma_t*, const void*,
+                      const void*, const void*, const void*, void*, void*, hipStream_t);
 };
 
 template <typename R, int NX, int NY, bool SQRT>
@@ -535,11 +541,58 @@ struct Impl {
     return hip_check("k_sample_apply");
   }
 
+  // Sequential passes (one lane per sequence); real dtypes only.
+  static constexpr bool kReal = std::is_same<R, float>::value || std::is_same<R, double>::value;
+  static int seq_filter(const psm_problem_t* p, const psm_model_t* tr, const psm_model_t* ob,
+                        const psm_sigma_t* sg, const void* ys, const void* m0, const void* L0,
+                        const void* nm, const void* nL, void* om, void* oL, double* ell,
+                        hipStream_t st) {
+    if constexpr (!kReal) {
+      set_error("sequential passes support PSM_F32 and PSM_F64 only");
+      return PSM_E_UNSUPPORTED;
+    } else {
+      if (p->n_seq <= 0) return PSM_OK;
+      int use = 0;
+      SigmaView<R> sv = sigma_view<R>(sg, &use);
+      ModelView<R> trv = model_view<R>(tr), obv = model_view<R>(ob);
+      trv.npts = use ? sv.npts : 0;
+      obv.npts = use ? sv.npts : 0;
+      const Fallback fb{p->fallback_ratio, p->fallback_count};
+      TimedScope ts_("k_seq_filter", st);
+      hipLaunchKernelGGL((k_seq_filter<R, NX, NY, SQRT>), dim3(grid_for(p->n_seq, kLaneWG)),
+                         dim3(kLaneWG), 0, st, p->n_seq, p->state_offsets, trv, obv, sv, use,
+                         (const R*)nm, (const R*)nL, (const R*)ys, (const R*)m0, (const R*)L0,
+                         (R*)om, (R*)oL, ell, fb);
+      return hip_check("k_seq_filter");
+    }
+  }
+  static int seq_smoother(const psm_problem_t* p, const psm_model_t* tr, const psm_sigma_t* sg,
+                          const void* fm, const void* fL, const void* nm, const void* nL, void* om,
+                          void* oL, hipStream_t st) {
+    if constexpr (!kReal) {
+      set_error("sequential passes support PSM_F32 and PSM_F64 only");
+      return PSM_E_UNSUPPORTED;
+    } else {
+      if (p->n_seq <= 0) return PSM_OK;
+      int use = 0;
+      SigmaView<R> sv = sigma_view<R>(sg, &use);
+      ModelView<R> trv = model_view<R>(tr);
+      trv.npts = use ? sv.npts : 0;
+      const Fallback fb{p->fallback_ratio, p->fallback_count};
+      TimedScope ts_("k_seq_smoother", st);
+      hipLaunchKernelGGL((k_seq_smoother<R, NX, SQRT>), dim3(grid_for(p->n_seq, kLaneWG)),
+                         dim3(kLaneWG), 0, st, p->n_seq, p->state_offsets, trv, sv, use,
+                         (const R*)nm, (const R*)nL, (const R*)fm, (const R*)fL, (R*)om, (R*)oL,
+                         fb);
+      return hip_check("k_seq_smoother");
+    }
+  }
+
   static const OpsTable* table() {
     static const OpsTable t = {&linearize_trans, &linearize_obs, &filter_elements,
                                &filter_combine,  &smoother_elements, &smoother_combine,
                                &workspace_bytes, &filter,          &smoother,
-                               &sample_workspace_bytes, &sample};
+                               &sample_workspace_bytes, &sample, &seq_filter, &seq_smoother};
     return &t;
   }
 };

@@ -224,6 +224,46 @@ def filter_smoother_pass(prob: Problem, mods: Models, ys, m0, L0, nom_m, nom_L, 
     return om, oL
 
 
+def _seq_models_ready(mods: Models, nom_m, nom_L, obs: bool = True):
+    if nom_m is None:
+        if mods.tabulated:
+            raise NotImplementedError(
+                "parallel=False without a nominal trajectory linearizes at the running estimate, "
+                "which a torch-callable or conditional-moments model cannot be evaluated at ahead "
+                "of the recursion; pass a nominal trajectory or use a parsmooth.models function")
+    else:
+        mods.prepare(nom_m, nom_L, obs=obs)
+
+
+def seq_filter_pass(prob: Problem, mods: Models, ys, m0, L0, nom_m, nom_L, want_ell: bool,
+                    out=None):
+    """Sequential filter, one lane per sequence (parallel=False).  nom_m None:
+    the transition is linearized at the filtered and the observation at the
+    predicted state."""
+    out_m, out_L = out if out is not None else prob.new_traj()
+    ell = torch.empty(prob.n_seq, dtype=torch.float64, device=prob.dev) if want_ell else None
+    _seq_models_ready(mods, nom_m, nom_L)
+    rc = N.lib().psm_seq_filter(ctypes.byref(prob.struct), ctypes.byref(mods.trans),
+                                ctypes.byref(mods.obs), mods.sigma_ref(), N.ptr(ys), N.ptr(m0),
+                                N.ptr(L0), N.ptr(nom_m), N.ptr(nom_L), N.ptr(out_m), N.ptr(out_L),
+                                N.ptr(ell), N.stream_ptr())
+    N.check(rc, "psm_seq_filter")
+    return out_m, out_L, ell
+
+
+def seq_smoother_pass(prob: Problem, mods: Models, fm, fL, nom_m, nom_L, out=None):
+    """Sequential (Rauch-Tung-Striebel) smoother, one lane per sequence.
+    nom_m None: the transition out of state k is linearized at the smoothed
+    state k+1."""
+    out_m, out_L = out if out is not None else prob.new_traj()
+    _seq_models_ready(mods, nom_m, nom_L, obs=False)
+    rc = N.lib().psm_seq_smoother(ctypes.byref(prob.struct), ctypes.byref(mods.trans), mods.sigma_ref(),
+                                  N.ptr(fm), N.ptr(fL), N.ptr(nom_m), N.ptr(nom_L), N.ptr(out_m),
+                                  N.ptr(out_L), N.stream_ptr())
+    N.check(rc, "psm_seq_smoother")
+    return out_m, out_L
+
+
 def mean_sq_diff(a: torch.Tensor, b: torch.Tensor) -> torch.Tensor:
     """mean((a - b)^2) on the device, accumulated in FP64 (one-element tensor)."""
     out = torch.empty(1, dtype=torch.float64, device=a.device)

@@ -38,7 +38,7 @@ EXPORTS = (
     "psm_filter_combine", "psm_smoother_elements", "psm_smoother_combine", "psm_workspace_bytes",
     "psm_filter", "psm_smoother", "psm_filter_smoother", "psm_mean_sq_diff", "psm_clear_graphs",
     "psm_timing_enable", "psm_timing_report", "psm_chunk_size", "psm_autotune",
-    "psm_sample_workspace_bytes", "psm_sample",
+    "psm_sample_workspace_bytes", "psm_sample", "psm_seq_filter", "psm_seq_smoother",
 )
 
 
@@ -94,6 +94,11 @@ _SIGS = {
     "psm_sample_workspace_bytes": (ctypes.c_size_t, [ctypes.POINTER(PsmProblem), _I32]),
     "psm_sample": (ctypes.c_int, [ctypes.POINTER(PsmProblem), ctypes.POINTER(PsmModel),
                                   ctypes.POINTER(PsmSigma)] + [_P] * 5 + [_I32, _P, _P, ctypes.c_size_t, _P]),
+    "psm_seq_filter": (ctypes.c_int, [ctypes.POINTER(PsmProblem), ctypes.POINTER(PsmModel),
+                                      ctypes.POINTER(PsmModel), ctypes.POINTER(PsmSigma)]
+                       + [_P] * 8 + [_P]),
+    "psm_seq_smoother": (ctypes.c_int, [ctypes.POINTER(PsmProblem), ctypes.POINTER(PsmModel),
+                                        ctypes.POINTER(PsmSigma)] + [_P] * 6 + [_P]),
     "psm_autotune": (ctypes.c_int, [ctypes.POINTER(PsmProblem), ctypes.POINTER(PsmModel),
                                     ctypes.POINTER(PsmModel), ctypes.POINTER(PsmSigma)]
                      + [_P] * 10 + [ctypes.c_size_t, _P, ctypes.POINTER(_I32), ctypes.POINTER(_I32),

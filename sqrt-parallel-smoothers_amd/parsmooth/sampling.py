@@ -7,6 +7,13 @@ a ``torch.Generator`` (on the device) or the normals themselves, a tensor of
 shape (T + 1, n_samples, nx) whose row k drives state k (row T: the last
 state).  Samples are returned with shape (T + 1, n_samples, nx), as the
 reference's.
+
+``parallel`` selects how the default nominal trajectory (the smoothed one) is
+computed.  The reference's sequential and parallel samplers share their gains,
+increments and normals (_pathwise_sampler.py:41-61 and :13-38 both start from
+``_sampling_common``) and differ only in evaluating the same backward affine
+recursion step by step or as a scan; the device evaluates it with its chunked
+scan either way.
 """
 from __future__ import annotations
 
@@ -20,6 +27,7 @@ from ._base import MVNSqrt, MVNStandard, are_inputs_compatible
 from ._engine import Models, Problem, to_dev
 from .linearization._methods import resolve
 from .parallel._smoothing import smoothing as _par_smoothing
+from .sequential._smoothing import smoothing as _seq_smoothing
 
 
 def _normals(key, shape, dtype, dev) -> torch.Tensor:
@@ -38,11 +46,10 @@ def _normals(key, shape, dtype, dev) -> torch.Tensor:
 
 def sampling(key, n_samples: int, transition_model, filter_trajectory: Union[MVNSqrt, MVNStandard],
              linearization_method: Callable, nominal_trajectory=None, parallel: bool = True):
-    if not parallel:
-        raise NotImplementedError("only the parallel formulation (parallel=True) runs on the HIP device")
     # the reference's default nominal is the smoothed trajectory (methods.py:86-87)
     if nominal_trajectory is None:
-        nominal_trajectory = _par_smoothing(transition_model, filter_trajectory, linearization_method, None)
+        smooth = _par_smoothing if parallel else _seq_smoothing
+        nominal_trajectory = smooth(transition_model, filter_trajectory, linearization_method, None)
     are_inputs_compatible(filter_trajectory, nominal_trajectory)
     sqrt = isinstance(filter_trajectory, MVNSqrt)
     fm0 = filter_trajectory[0]

@@ -129,3 +129,29 @@ def test_schedule_fields_change_the_plan():
     rc = lib.psm_autotune(ctypes.byref(p), ctypes.byref(m), ctypes.byref(m), None, *([None] * 10), 0, None,
                           None, None, None)
     assert rc != 0 and b"null" in lib.psm_last_error()
+
+
+def test_sequential_entry_points_check_arguments():
+    """psm_seq_filter / psm_seq_smoother refuse, before any launch: a nominal
+    mean without its chol (or the reverse), tangent dtypes, caller-evaluated
+    models without a nominal, and null arrays."""
+    lib = N.load_library()
+    p = N.PsmProblem(N.PSM_F64, N.FORM_SQRT, 5, 2, 1, 0, 11, 1)  # dummy non-null offsets pointer
+    lin = N.PsmModel(N.MODEL_LINEAR, 5, 5, 0, 1, 1, 1)
+    obs = N.PsmModel(N.MODEL_LINEAR, 5, 2, 0, 1, 1, 1)
+    tab = N.PsmModel(N.MODEL_TABULATED, 5, 5, 0, 1, 1, 1)
+    one = ctypes.c_void_p(1)
+    rc = lib.psm_seq_filter(ctypes.byref(p), ctypes.byref(lin), ctypes.byref(obs), None, one, one, one,
+                            one, None, one, one, None, None)
+    assert rc == -1 and b"nom_m and nom_L" in lib.psm_last_error()
+    rc = lib.psm_seq_filter(ctypes.byref(p), ctypes.byref(tab), ctypes.byref(obs), None, one, one, one,
+                            None, None, one, one, None, None)
+    assert rc == -1 and b"nominal" in lib.psm_last_error()
+    rc = lib.psm_seq_filter(ctypes.byref(p), ctypes.byref(lin), ctypes.byref(obs), None, None, one, one,
+                            None, None, one, one, None, None)
+    assert rc == -1 and b"null" in lib.psm_last_error()
+    rc = lib.psm_seq_smoother(ctypes.byref(p), ctypes.byref(tab), None, one, one, None, None, one, one, None)
+    assert rc == -1 and b"nominal" in lib.psm_last_error()
+    pd = N.PsmProblem(N.PSM_F64_TANGENT, N.FORM_SQRT, 5, 2, 1, 0, 11, 1)
+    rc = lib.psm_seq_smoother(ctypes.byref(pd), ctypes.byref(lin), None, one, one, one, one, one, one, None)
+    assert rc == -1 and b"PSM_F32 or PSM_F64" in lib.psm_last_error()

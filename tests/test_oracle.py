@@ -674,3 +674,122 @@ def test_get_conditional_model_lognormal(sqrt):
     np.testing.assert_allclose(mean, x * np.exp(m + s * s / 2), rtol=1e-11)
     np.testing.assert_allclose(var, x * x * np.expm1(s * s) * np.exp(2 * m + s * s), rtol=1e-10)
     np.testing.assert_allclose(model.conditional_mean.jac(xs)[:, 0, 0], np.exp(m + s * s / 2), rtol=1e-11)
+
+
+# ------------------------------------- sequential passes on non-linear models
+def _nonlinear_problem(kind, sqrt, T=25, seed=3):
+    """Simulated data and (x0, transition, observation) for a non-linear model:
+    'ct_bearings' (coordinated turn + two bearings), 'ct_linear' (coordinated
+    turn observed in position) or 'cv_bearings' (constant velocity + bearings)."""
+    rng = np.random.RandomState(seed)
+    dt = 0.1
+    Q5, R = O.bearings_parameters(0.1, 0.2, 0.05, dt)
+    if kind == "cv_bearings":
+        F = np.eye(4)
+        F[0, 2] = F[1, 3] = dt
+        Q = Q5[:4, :4]
+        f = O.Linear(F)
+        m0 = np.array([0.1, 0.2, 1.0, 0.5])
+    else:
+        Q = Q5
+        f = O.CoordinatedTurn(dt)
+        m0 = np.array([0.1, 0.2, 1.0, 0.5, 0.8])
+    nx = m0.size
+    if kind == "ct_linear":
+        H = np.zeros((2, nx))
+        H[0, 0] = H[1, 1] = 1.0
+        h = O.Linear(H)
+    else:
+        h = O.Bearings((-1.5, 0.5), (1.0, 1.0))
+    cQ, cR = np.linalg.cholesky(Q), np.linalg.cholesky(R)
+    x = m0.copy()
+    ys = np.empty((T, 2))
+    for k in range(T):
+        x = f(x) + cQ @ rng.randn(nx)
+        ys[k] = h(x) + cR @ rng.randn(2)
+    P0 = 0.1 * np.eye(nx)
+    if sqrt:
+        mk = lambda M, C: O.MVNSqrt(M, np.linalg.cholesky(C))
+    else:
+        mk = O.MVNStandard
+    return ys, mk(m0, P0), O.FunctionalModel(f, mk(np.zeros(nx), Q)), O.FunctionalModel(h, mk(np.zeros(2), R))
+
+
+def _noisy_nominal(rng, traj, sqrt):
+    m = traj[0] + 0.05 * rng.randn(*traj[0].shape)
+    L = np.tile(0.3 * np.eye(m.shape[1]), (m.shape[0], 1, 1))
+    return O.MVNSqrt(m, L) if sqrt else O.MVNStandard(m, L @ L)
+
+
+@pytest.mark.parametrize("method", list(METHODS))
+@pytest.mark.parametrize("sqrt", [True, False])
+def test_sequential_equals_parallel_at_a_nominal(method, sqrt):
+    """At a given nominal trajectory both formulations linearize at the same
+    points, so the sequential recursion and the parallel scan give the same
+    filter, log-likelihood and smoother on a non-linear model."""
+    ys, x0, tr, ob = _nonlinear_problem("ct_bearings", sqrt)
+    meth = METHODS[method]
+    rng = np.random.RandomState(1)
+    nom = _noisy_nominal(rng, O.filtering(ys, x0, tr, ob, meth, None, False), sqrt)
+    fs, es = O.filtering(ys, x0, tr, ob, meth, nom, False, return_loglikelihood=True)
+    fp, ep = O.filtering(ys, x0, tr, ob, meth, nom, True, return_loglikelihood=True)
+    np.testing.assert_allclose(fs[0], fp[0], rtol=1e-9, atol=1e-9)
+    np.testing.assert_allclose(fs[1], fp[1], rtol=1e-9, atol=1e-9)
+    assert es == pytest.approx(ep, rel=1e-9)
+    ss = O.smoothing(tr, fs, meth, nom, False)
+    sp = O.smoothing(tr, fs, meth, nom, True)
+    np.testing.assert_allclose(ss[0], sp[0], rtol=1e-9, atol=1e-9)
+    np.testing.assert_allclose(ss[1], sp[1], rtol=1e-9, atol=1e-9)
+
+
+def _predicted(tr, meth, traj, sqrt):
+    """The predicted distributions F x_{k-1} + b (chol tria([F L, cq]) or
+    covariance F P F^T + Q) of a linear transition from every state but the last."""
+    F, cq, b = O.linearize_many(meth, tr, type(traj)(traj[0][:-1], traj[1][:-1]))
+    m = np.einsum("kij,kj->ki", F, traj[0][:-1]) + b
+    if sqrt:
+        L = O.tria(np.concatenate([F @ traj[1][:-1], cq], -1))
+    else:
+        L = F @ traj[1][:-1] @ np.swapaxes(F, -1, -2) + cq
+    return m, L
+
+
+@pytest.mark.parametrize("method", list(METHODS))
+@pytest.mark.parametrize("sqrt", [True, False])
+def test_sequential_linearization_points(method, sqrt):
+    """Without a nominal trajectory the sequential filter linearizes the
+    transition at the filtered state k-1 and the observation at the predicted
+    state k, and the smoother the transition out of k at the smoothed state k+1
+    (parsmooth/sequential/_filtering.py:36-41, _smoothing.py:27-33).  Each
+    result is therefore reproduced by the same pass at the nominal trajectory
+    made of those points: with a linear observation the filter at its own
+    output; with a linear transition the filter at its own predictions; the
+    smoother at its output shifted by one state."""
+    meth = METHODS[method]
+    # transition points: a linear observation makes the observation point irrelevant
+    ys, x0, tr, ob = _nonlinear_problem("ct_linear", sqrt)
+    filt = O.filtering(ys, x0, tr, ob, meth, None, False)
+    again = O.filtering(ys, x0, tr, ob, meth, filt, False)
+    np.testing.assert_allclose(again[0], filt[0], rtol=1e-10, atol=1e-11)
+    np.testing.assert_allclose(again[1], filt[1], rtol=1e-10, atol=1e-11)
+    # observation points: a linear transition makes the transition point irrelevant
+    ys, x0, tr, ob = _nonlinear_problem("cv_bearings", sqrt)
+    filt = O.filtering(ys, x0, tr, ob, meth, None, False)
+    pm, pL = _predicted(tr, meth, filt, sqrt)
+    nom = type(filt)(np.concatenate([filt[0][:1], pm]), np.concatenate([filt[1][:1], pL]))
+    again = O.filtering(ys, x0, tr, ob, meth, nom, False)
+    np.testing.assert_allclose(again[0], filt[0], rtol=1e-10, atol=1e-11)
+    np.testing.assert_allclose(again[1], filt[1], rtol=1e-10, atol=1e-11)
+    # and at the filtered states instead the result changes (the points matter here)
+    other = O.filtering(ys, x0, tr, ob, meth, filt, False)
+    assert np.abs(other[0] - filt[0]).max() > 1e-6
+    # smoother points: the smoothed successor
+    ys, x0, tr, ob = _nonlinear_problem("ct_bearings", sqrt)
+    filt = O.filtering(ys, x0, tr, ob, meth, None, False)
+    sm = O.smoothing(tr, filt, meth, None, False)
+    shifted = type(sm)(np.concatenate([sm[0][1:], sm[0][-1:]]), np.concatenate([sm[1][1:], sm[1][-1:]]))
+    again = O.smoothing(tr, filt, meth, shifted, False)
+    np.testing.assert_allclose(again[0], sm[0], rtol=1e-10, atol=1e-11)
+    np.testing.assert_allclose(again[1], sm[1], rtol=1e-10, atol=1e-11)
+    other = O.smoothing(tr, filt, meth, sm, False)
+    assert np.abs(other[0] - sm[0]).max() > 1e-6
