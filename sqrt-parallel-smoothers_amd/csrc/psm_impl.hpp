@@ -318,11 +318,13 @@ struct Impl {
   }
 
   // Exclusive scan of the level-0 items (chunk aggregates, already in scan
-  // order in the interleaved layout); the complete prefixes end up at
-  // *final_off, in the same layout.
+  // order in the interleaved layout).  The level-0 down-sweep is left to the
+  // apply kernel (level0_prefix): *final_off receives the level-0 prefixes
+  // within each workgroup and *upper_off the complete level-1 prefixes
+  // (SIZE_MAX with a single level, whose prefixes are already complete).
   template <typename Agg>
   static int run_scan(const Plan& pl, char* ws, const Fallback& fb, hipStream_t st,
-                      size_t* final_off) {
+                      size_t* final_off, size_t* upper_off) {
     constexpr bool filt = std::is_same<Agg, FA>::value;
     for (int l = 0; l < pl.nlevels; ++l) {
       const ScanLevel& s = pl.lv[l];
@@ -355,8 +357,8 @@ struct Impl {
       int rc = hip_check("k_scan_up");
       if (rc) return rc;
     }
-    // complete prefixes of level l land in lv[l].in (free once the up-sweep is done)
-    for (int l = pl.nlevels - 2; l >= 0; --l) {
+    // complete prefixes of level l >= 1 land in lv[l].in (free once the up-sweep is done)
+    for (int l = pl.nlevels - 2; l >= 1; --l) {
       const ScanLevel& s = pl.lv[l];
       const size_t upper_off =
           (l + 1 == pl.nlevels - 1) ? pl.lv[l + 1].off_out : pl.lv[l + 1].off_in;
@@ -380,7 +382,9 @@ struct Impl {
       int rc = hip_check("k_scan_down");
       if (rc) return rc;
     }
-    *final_off = pl.nlevels == 1 ? pl.lv[0].off_out : pl.lv[0].off_in;
+    *final_off = pl.lv[0].off_out;
+    *upper_off = pl.nlevels == 1 ? SIZE_MAX
+                                 : (pl.nlevels == 2 ? pl.lv[1].off_out : pl.lv[1].off_in);
     return PSM_OK;
   }
 
@@ -427,16 +431,18 @@ struct Impl {
                        (R*)(ws + pl.lv[0].off_in), level_map(pl, 0, 0), fb);
     }
     if ((rc = hip_check("k_filter_reduce"))) return rc;
-    size_t pre_off = 0;
-    if ((rc = run_scan<FA>(pl, ws, fb, st, &pre_off))) return rc;
+    size_t pre_off = 0, up_off = 0;
+    if ((rc = run_scan<FA>(pl, ws, fb, st, &pre_off, &up_off))) return rc;
     Acc* ell_state = ell ? (Acc*)(ws + pl.off_ell) : nullptr;
     {
     TimedScope ts_("k_filter_apply", st);
     hipLaunchKernelGGL((k_filter_apply<R, NX, NY, SQRT>), dim3(grid_for(pl.nchunks, kLaneWG)),
                        dim3(kLaneWG), 0, st, pl.M, pl.Mp, pl.chunk, pl.nchunks, p->state_offsets,
                        p->n_seq, lin, (const R*)ys, (const R*)m0, (const R*)L0,
-                       (const R*)(ws + pre_off), level_map(pl, 0, 0), (R*)om, (R*)oL, ell_state,
-                       fb);
+                       (const R*)(ws + pre_off), level_map(pl, 0, 0),
+                       up_off == SIZE_MAX ? nullptr : (const R*)(ws + up_off),
+                       pl.nlevels > 1 ? level_map(pl, 1, 0) : level_map(pl, 0, 0),
+                       (R*)(ws + pl.lv[0].off_in), (R*)om, (R*)oL, ell_state, fb);
     }
     if ((rc = hip_check("k_filter_apply"))) return rc;
     if (ell) {
@@ -472,14 +478,16 @@ struct Impl {
                        fb);
     }
     if ((rc = hip_check("k_smoother_reduce"))) return rc;
-    size_t pre_off = 0;
-    if ((rc = run_scan<SA>(pl, ws, fb, st, &pre_off))) return rc;
+    size_t pre_off = 0, up_off = 0;
+    if ((rc = run_scan<SA>(pl, ws, fb, st, &pre_off, &up_off))) return rc;
     {
       TimedScope ts_("k_smoother_apply", st);
       hipLaunchKernelGGL((k_smoother_apply<R, NX, SQRT>), dim3(grid_for(pl.nchunks, kLaneWG)),
                          dim3(kLaneWG), 0, st, pl.M, pl.Mp, pl.chunk, pl.nchunks,
                          (const R*)(ws + pl.off_selem), (const R*)(ws + pre_off),
-                         level_map(pl, 0, 1), (R*)om, (R*)oL, fb);
+                         level_map(pl, 0, 1), up_off == SIZE_MAX ? nullptr : (const R*)(ws + up_off),
+                         pl.nlevels > 1 ? level_map(pl, 1, 0) : level_map(pl, 0, 0),
+                         (R*)(ws + pl.lv[0].off_in), (R*)om, (R*)oL, fb);
     }
     return hip_check("k_smoother_apply");
   }

@@ -468,11 +468,35 @@ __global__ __launch_bounds__((TeamCfg<Agg, R>::WGT)) void k_scan_down_team(
 // ============================================================================
 // 4. filter application
 // ============================================================================
+// Complete exclusive prefix of level-0 item c (chunk c, in scan order when
+// lm.rev).  The level-0 up-sweep leaves each item's prefix within its
+// workgroup; items of workgroup w > 0 still need the prefix of the workgroups
+// before them (level 1: upper[w]).  That last combination -- what a level-0
+// down-sweep would do for every item -- is done here by the lane that uses it,
+// into joined (the free level-0 input buffer).  upper == nullptr: a one-level
+// scan, whose prefixes are already complete.  Returns the buffer holding the
+// prefix and its position *pc.
+template <typename Agg, typename R, bool SQRT>
+PSM_DEV const R* level0_prefix(int64_t c, const R* __restrict__ prefix, const LevelMap& lm,
+                               const R* __restrict__ upper, const LevelMap& um,
+                               R* __restrict__ joined, const Fallback& fb, int64_t* pc) {
+  using V = typename ViewOf<Agg>::type;
+  constexpr int NX = Agg_nx<Agg>::value;
+  const int64_t p = lm.rev ? lm.n - 1 - c : c;
+  *pc = lm.at(p);
+  const int64_t w = p / ((int64_t)lm.K * lm.WG);
+  if (upper == nullptr || w == 0) return prefix;
+  combine_mem_fb<R, NX, SQRT>(V{const_cast<R*>(upper), um.stride, um.at(w)},
+                              V{const_cast<R*>(prefix), lm.stride, *pc}, V{joined, lm.stride, *pc}, fb);
+  return joined;
+}
+
 template <typename R, int NX, int NY, bool SQRT>
 __global__ __launch_bounds__(kLaneWG) void k_filter_apply(
     int64_t M, int64_t Mp, int chunk, int64_t nchunks, const int64_t* __restrict__ soff, int nseq,
     const R* __restrict__ lin, const R* __restrict__ ys, const R* __restrict__ m0,
-    const R* __restrict__ L0, const R* __restrict__ prefix, LevelMap lm, R* __restrict__ out_m,
+    const R* __restrict__ L0, const R* __restrict__ prefix, LevelMap lm,
+    const R* __restrict__ upper, LevelMap um, R* __restrict__ joined, R* __restrict__ out_m,
     R* __restrict__ out_L, typename AccOf<R>::type* __restrict__ ell_state, Fallback fb) {
   using Acc = typename AccOf<R>::type;
   const StateMap sm{chunk};
@@ -483,11 +507,12 @@ __global__ __launch_bounds__(kLaneWG) void k_filter_apply(
   int s = seq_of(soff, nseq, i0);
   R m[NX], L[NX][NX];
   {
-    // incoming filtered state: (b, U) of the exclusive prefix
+    // incoming filtered state: (b, U) of the complete exclusive prefix
+    int64_t pc;
+    const R* src = level0_prefix<FAgg<R, NX>, R, SQRT>(c, prefix, lm, upper, um, joined, fb, &pc);
     int q = NX * NX;
-    const int64_t pc = lm.pos(c);
-    ld(prefix, lm.stride, pc, q, m);
-    ld(prefix, lm.stride, pc, q, L);
+    ld(src, lm.stride, pc, q, m);
+    ld(src, lm.stride, pc, q, L);
   }
   for (int64_t i = i0; i < i1; ++i) {
     while (i >= soff[s + 1]) ++s;
@@ -574,8 +599,8 @@ __global__ __launch_bounds__(kLaneWG) void k_smoother_reduce(
 template <typename R, int NX, bool SQRT>
 __global__ __launch_bounds__(kLaneWG) void k_smoother_apply(
     int64_t M, int64_t Mp, int chunk, int64_t nchunks, const R* __restrict__ elems,
-    const R* __restrict__ prefix, LevelMap lm, R* __restrict__ out_m, R* __restrict__ out_L,
-    Fallback fb) {
+    const R* __restrict__ prefix, LevelMap lm, const R* __restrict__ upper, LevelMap um,
+    R* __restrict__ joined, R* __restrict__ out_m, R* __restrict__ out_L, Fallback fb) {
   const StateMap sm{chunk};
   const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= nchunks) return;
@@ -583,11 +608,12 @@ __global__ __launch_bounds__(kLaneWG) void k_smoother_apply(
   const int64_t i1 = min(i0 + chunk, M);
   R m[NX], D[NX][NX];
   {
+    int64_t pc;
+    const R* src = level0_prefix<SAgg<R, NX>, R, SQRT>(c, prefix, lm, upper, um, joined, fb, &pc);
     int q = 0;
-    const int64_t pc = lm.pos(c);
-    ld(prefix, lm.stride, pc, q, m);
+    ld(src, lm.stride, pc, q, m);
     q += NX * NX;
-    ld(prefix, lm.stride, pc, q, D);
+    ld(src, lm.stride, pc, q, D);
   }
   for (int64_t i = i1 - 1; i >= i0; --i) {
     SAgg<R, NX> e;
