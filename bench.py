@@ -382,8 +382,14 @@ def widening_legs(steps, warmup):
     mean, _ = time_calls(lambda: filter_smoother_pass(prob, mods, yb, m0b, L0b, nomb[0], nomb[1], fb_, ob_),
                          steps, warmup)
     row["batch_parallel_steps_per_s"] = nb * Tb / mean
+    # the batch runner's autotuner makes the same choice by timing both at a nominal
+    from parsmooth.batch import IteratedSmoother
+    rb = IteratedSmoother([Tb] * nb, *builtin, extended)
+    rb.set_data(yb, m0b, L0b)
+    rb.autotune()
+    row["batch_runner_autotune"] = rb.tuned
     res["sequential_passes"] = row
-    del prob, mods, yb, nomb, fb_, ob_
+    del prob, mods, yb, nomb, fb_, ob_, rb
 
     # conditional-moments models: population model, cubature IPLS
     rng = np.random.RandomState(3)

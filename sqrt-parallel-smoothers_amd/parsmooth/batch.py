@@ -16,7 +16,8 @@ import torch
 
 from . import _native as N
 from ._base import ConditionalMomentsModel, MVNSqrt, MVNStandard
-from ._engine import Models, Problem, filter_pass, filter_smoother_pass, to_dev
+from ._engine import (Models, Problem, filter_pass, filter_smoother_pass, seq_filter_pass,
+                      seq_smoother_pass, to_dev)
 from .linearization._methods import resolve
 
 
@@ -26,6 +27,11 @@ def _noise_dim(model) -> int:
         raise TypeError("pass nx (and ny) explicitly for conditional-moments models")
     q = model[1][0]
     return int(q.numel()) if isinstance(q, torch.Tensor) else int(np.prod(np.shape(q)))
+
+
+# Lower bound of the sequential filter + smoother time per step of one lane
+# (measured: ~18 us in FP64 on MI355X for the bearings model).
+SEQ_STEP_MS_MIN = 0.004
 
 
 class IteratedSmoother:
@@ -38,11 +44,17 @@ class IteratedSmoother:
     linearization_method: parsmooth.linearization.extended / cubature / ...
     dtype: torch.float64 or torch.float32.
     sqrt: square-root (MVNSqrt) or covariance (MVNStandard) form.
+    sequential: run each pass as the sequential recursion, one lane per
+        sequence (psm_seq_filter / psm_seq_smoother at the same nominal
+        trajectory: the same posterior as the parallel passes, to rounding).
+        Faster for batches of many short sequences; ``autotune`` chooses.
     """
 
     def __init__(self, lengths: Sequence[int], transition_model, observation_model,
                  linearization_method: Callable, dtype=torch.float64, sqrt: bool = True, chunk: int = 0,
-                 nx: int = None, ny: int = None):
+                 nx: int = None, ny: int = None, sequential: bool = False):
+        self.sequential = bool(sequential)
+        self.tuned = None
         self.dev = N.device()
         self.dtype = dtype
         self.sqrt = sqrt
@@ -63,11 +75,14 @@ class IteratedSmoother:
         """Number of FP64 recomputations since construction (FP32 square-root form)."""
         return int(self.prob.fallback_counter.item())
 
-    def autotune(self, nominal=None):
+    def autotune(self, nominal=None, reps: int = 3):
         """Pick the fastest launch schedule (chunk, scan fan-in) for this batch
         by timing one filter-smoother pass per candidate at ``nominal`` (the
-        default nominal when None); returns (chunk, scan_k, ms per pass).  Cached
-        graphs are dropped, since the schedule changes the launch sequence."""
+        default nominal when None); returns (chunk, scan_k, ms per pass) of the
+        parallel schedule.  The sequential passes (one lane per sequence) are
+        timed at the same nominal too and replace the parallel ones when faster;
+        ``self.tuned`` records both times and the choice.  Cached graphs are
+        dropped, since the schedule changes the launch sequence."""
         if self.ys is None:
             raise RuntimeError("call set_data first")
         if nominal is None:
@@ -78,7 +93,32 @@ class IteratedSmoother:
         res = self.prob.autotune(self.mods, self.ys, self.m0, self.L0, nominal[0], nominal[1], self._filt,
                                  self._bufs[1])
         N.lib().psm_clear_graphs()
+        par_ms = self._time_pass(nominal, False, reps)
+        # the sequential passes cost at least SEQ_STEP_MS_MIN per step of the
+        # longest sequence; skip timing them when that alone loses
+        seq_ms = None
+        if int(self.prob.lengths.max()) * SEQ_STEP_MS_MIN < par_ms:
+            seq_ms = self._time_pass(nominal, True, reps)
+        self.sequential = seq_ms is not None and seq_ms < par_ms
+        self.tuned = {"chunk": int(res[0]), "scan_k": int(res[1]), "parallel_ms": par_ms,
+                      "sequential_ms": seq_ms, "sequential": self.sequential}
         return res
+
+    def _time_pass(self, nominal, sequential: bool, reps: int) -> float:
+        """Mean ms of one filter-smoother pass at ``nominal`` (after one untimed pass)."""
+        keep = self.sequential
+        self.sequential = sequential
+        try:
+            self.pass_once(nominal, self._bufs[1])
+            start, end = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            start.record()
+            for _ in range(reps):
+                self.pass_once(nominal, self._bufs[1])
+            end.record()
+            end.synchronize()
+            return start.elapsed_time(end) / reps
+        finally:
+            self.sequential = keep
 
     def chunk_size(self) -> int:
         """States per lane in the chunk passes (as resolved by the library)."""
@@ -98,6 +138,12 @@ class IteratedSmoother:
         self.L0 = to_dev(L0, self.dtype, self.dev).reshape(p.n_seq, self.nx, self.nx)
 
     def pass_once(self, nominal, out, use_graph=False):
+        if self.sequential:
+            seq_filter_pass(self.prob, self.mods, self.ys, self.m0, self.L0, nominal[0], nominal[1], False,
+                            out=self._filt)
+            seq_smoother_pass(self.prob, self.mods, self._filt[0], self._filt[1], nominal[0], nominal[1],
+                              out=out)
+            return out
         filter_smoother_pass(self.prob, self.mods, self.ys, self.m0, self.L0, nominal[0], nominal[1],
                              self._filt, out, use_graph)
         return out
@@ -124,8 +170,9 @@ class IteratedSmoother:
             cur = self.pass_once(cur, nxt, use_graph)
             k += 1
         if return_loglikelihood:
-            _, _, ell = filter_pass(self.prob, self.mods, self.ys, self.m0, self.L0, cur[0], cur[1], True,
-                                    out=self._filt)
+            run = seq_filter_pass if self.sequential else filter_pass
+            _, _, ell = run(self.prob, self.mods, self.ys, self.m0, self.L0, cur[0], cur[1], True,
+                            out=self._filt)
             return cur[0], cur[1], ell
         return cur[0], cur[1]
 

@@ -180,3 +180,56 @@ def test_sequential_sampling_default_nominal():
     got = parsmooth.sampling(t64(eps), S, _dev_model(tr), _dev_mvn(filt), extended, None, parallel=False)
     close(npy(got), ref, 1e-9, "samples")
 
+
+
+def _batch(lengths, sqrt, seed0):
+    from parsmooth.batch import IteratedSmoother
+    probs = [_nonlinear_problem("ct_bearings", sqrt, T=T, seed=seed0 + s) for s, T in enumerate(lengths)]
+    _, x0, tr, ob = probs[0]
+    ys = np.concatenate([p[0] for p in probs])
+    m0 = np.stack([p[1][0] for p in probs])
+    L0 = np.stack([p[1][1] for p in probs])
+    return probs, (tr, ob), ys, m0, L0, IteratedSmoother
+
+
+@pytest.mark.parametrize("sqrt", [True, False])
+def test_batch_runner_sequential_mode_matches_oracle(sqrt):
+    """IteratedSmoother(sequential=True): every pass at the previous smoothed
+    trajectory runs as the sequential recursion; the iterates and the final
+    log-likelihood equal the oracle's per sequence."""
+    lengths = [1, 7, 30, 2, 19, 64]
+    probs, (tr, ob), ys, m0, L0, IteratedSmoother = _batch(lengths, sqrt, 40)
+    r = IteratedSmoother(lengths, _dev_model(tr), _dev_model(ob), cubature, sqrt=sqrt, sequential=True)
+    r.set_data(t64(ys), t64(m0), t64(L0))
+    nom_m = np.concatenate([np.tile(p[1][0], (T + 1, 1)) for p, T in zip(probs, lengths)])
+    nom_L = np.concatenate([np.tile(p[1][1], (T + 1, 1, 1)) for p, T in zip(probs, lengths)])
+    gm, gL, ell = r.run(3, (t64(nom_m), t64(nom_L)), return_loglikelihood=True)
+    offs = r.prob.state_offsets_host
+    crit = lambda i, *_: i < 3
+    for s, p in enumerate(probs):
+        a, b = offs[s], offs[s + 1]
+        nom = type(p[1])(nom_m[a:b], nom_L[a:b])
+        ref, rell = O.iterated_smoothing(p[0], p[1], tr, ob, O.cubature, nom, False, criterion=crit,
+                                         return_loglikelihood=True)
+        close(npy(gm[a:b]), ref[0], 1e-9, f"seq {s} mean")
+        close(npy(gL[a:b]), ref[1], 1e-9, f"seq {s} chol/cov")
+        close(float(ell[s]), rell, 1e-9, f"seq {s} log-likelihood")
+
+
+def test_autotune_chooses_sequential_for_many_short_sequences():
+    """8192 sequences of 4 steps: one lane per sequence beats the chunked scan,
+    and the autotuned runner gives the same iterates as the parallel one."""
+    n, T = 8192, 4
+    ys1, x0, tr, ob = _nonlinear_problem("ct_bearings", True, T=T, seed=3)
+    from parsmooth.batch import IteratedSmoother
+    mk = lambda: IteratedSmoother([T] * n, _dev_model(tr), _dev_model(ob), extended)
+    r, ref_r = mk(), mk()
+    for rr in (r, ref_r):
+        rr.set_data(t64(np.tile(ys1, (n, 1))), t64(np.tile(x0[0], (n, 1))), t64(np.tile(x0[1], (n, 1, 1))))
+    r.autotune()
+    assert r.tuned["sequential"], r.tuned
+    assert r.tuned["sequential_ms"] < r.tuned["parallel_ms"]
+    got = r.run(3)
+    want = ref_r.run(3, use_graph=False)
+    close(npy(got[0]), npy(want[0]), 1e-9, "mean")
+    close(npy(got[1]), npy(want[1]), 1e-9, "chol")
