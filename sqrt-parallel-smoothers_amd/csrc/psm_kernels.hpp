@@ -19,11 +19,13 @@
 //                           by absorbing its steps one by one
 //   3. k_scan_up/k_scan_down exclusive scan of the chunk aggregates with the
 //                           full filtering operator (Kogge-Stone inside a
-//                           workgroup, recursive across workgroups)
-//   4. k_filter_apply       one lane per chunk: the sequential square-root
-//                           Kalman filter from the chunk's incoming filtered
-//                           state, writing the filtered states and the
-//                           per-step log-likelihood terms
+//                           workgroup, recursive across workgroups; no
+//                           level-0 down-sweep, see 4)
+//   4. k_filter_apply       one lane per chunk: joins the chunk's level-0
+//                           prefix with its workgroup's prefix, then runs the
+//                           sequential square-root Kalman filter from that
+//                           incoming filtered state, writing the filtered
+//                           states and the per-step log-likelihood terms
 // and the smoother pass mirrors it backwards in time (elements, reverse scan,
 // application).  Because the first state of every sequence carries its prior
 // (and the last state of every sequence is the smoother's terminal element),
