@@ -629,8 +629,8 @@ def main():
                     help="run one 256 MiB device copy first (byte-counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py runs on one GPU only (got --gpus {args.gpus}); multi-GPU runs are not part "
-              "of this build", file=sys.stderr)
+        print(f"bench.py runs on one GPU only (got --gpus {args.gpus}): the library has no "
+              "multi-GPU path", file=sys.stderr)
         sys.exit(2)
     import torch
     torch.cuda.set_device(0)
