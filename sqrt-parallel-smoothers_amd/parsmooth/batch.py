@@ -48,6 +48,7 @@ class IteratedSmoother:
         sequence (psm_seq_filter / psm_seq_smoother at the same nominal
         trajectory: the same posterior as the parallel passes, to rounding).
         Faster for batches of many short sequences; ``autotune`` chooses.
+        These passes are two plain launches each (``use_graph`` does not apply).
     """
 
     def __init__(self, lengths: Sequence[int], transition_model, observation_model,
