@@ -84,10 +84,19 @@ def get_conditional_model(f, q, linearization_method):
     sqrt = isinstance(q, MVNSqrt)
     method = resolve(linearization_method)
     consts = _NoiseConstants(q, sqrt, method)
-    probe = torch.as_tensor(consts.m, dtype=torch.float64)
-    try:
-        f(probe, probe)
-    except Exception:  # noqa: BLE001 - the reference catches everything here (:43-46)
+    # the reference's dimension probe (:43-46), on q's device and, if f keeps
+    # its constants on the device the kernels use, there too
+    devices = [q[0].device if isinstance(q[0], torch.Tensor) else torch.device("cpu")]
+    if devices[0].type != "cuda" and torch.cuda.is_available():
+        devices.append(torch.device("cuda", torch.cuda.current_device()))
+    for dev in devices:
+        probe = torch.as_tensor(consts.m, dtype=torch.float64, device=dev)
+        try:
+            f(probe, probe)
+            break
+        except Exception:  # noqa: BLE001 - the reference catches everything here
+            continue
+    else:
         raise NotImplementedError("`x` and `q` with different dimensions are not supported yet.")
 
     def _values(x):

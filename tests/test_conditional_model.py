@@ -147,3 +147,15 @@ def test_lognormal_gauss_hermite_order(sqrt):
 def test_rejects_non_gaussian_noise():
     with pytest.raises(TypeError):
         get_conditional_model(lognormal_torch, (torch.zeros(1), torch.ones(1, 1)), extended)
+
+
+def test_dimension_probe_runs_on_the_noise_device():
+    """The probe f(q.mean, q.mean) runs where q lives: a function whose constants
+    sit on that device passes it, and a dimension mismatch still raises."""
+    A = torch.eye(2, dtype=torch.float64)
+    f = lambda x, q: x @ A.T + q   # noqa: E731
+    q = MVNSqrt(torch.zeros(2, dtype=torch.float64), torch.eye(2, dtype=torch.float64))
+    get_conditional_model(f, q, cubature)
+    with pytest.raises(NotImplementedError):
+        get_conditional_model(f, MVNSqrt(torch.zeros(3, dtype=torch.float64), torch.eye(3, dtype=torch.float64)),
+                              cubature)

@@ -187,3 +187,21 @@ def test_noise_nonlinear_filter_smoother_matches_oracle(method, sqrt):
     ref = O.iterated_smoothing(ys, x0o, to, oo, OMETH[method], nomo, criterion=crit)
     close(npy(got[0]), ref[0], 1e-9, "iterated mean")
     close(npy(got[1]), ref[1], 1e-9, "iterated chol/cov")
+
+
+def test_get_conditional_model_with_device_constants():
+    """f keeps a constant on the device while q comes as numpy arrays: the
+    dimension probe finds the device, and the device linearization of the
+    resulting model equals the oracle's."""
+    rng = np.random.RandomState(5)
+    a, b = rng.randn(2, 2), 0.5 * np.eye(2)
+    A = t64(a)
+    f = lambda x, q: x @ A.T + 0.5 * q   # noqa: E731
+    cq = np.array([[0.3, 0.0], [0.1, 0.2]])
+    model = get_conditional_model(f, MVNSqrt(np.zeros(2), cq), cubature)
+    m, L = rng.randn(7, 2), np.tile(np.diag([0.4, 0.5]), (7, 1, 1))
+    got = cubature(model, MVNSqrt(t64(m), t64(L)))
+    ref = O.cubature(O.get_conditional_model(_LinXQ(a, b, np.zeros(2)), O.MVNSqrt(np.zeros(2), cq), O.cubature),
+                     O.MVNSqrt(m, L))
+    for u, v, name in zip(got, ref, "FCb"):
+        close(npy(u), v, 1e-10, f"device-constant model {name}")
