@@ -39,6 +39,27 @@ def bearings_data(x0, dt, r, T, s1, s2, q=10.0, random_state=None):
     return ts, true_states, observations
 
 
+def population_data(x0, T, Q, lam, random_state=None):
+    """Ricker-map states and Poisson counts (notebooks/population_model.py:172-208):
+    x_{k+1} = log(44.7) + x_k - exp(x_k) + chol(Q) eps_k, y_k ~ Poisson(lam exp(x_{k+1})).
+    Returns (true_states (T+1, 1) with x0 first, observations (T, 1) as float).
+    The reference draws from JAX keys; this draws from numpy's
+    RandomState(random_state), so the same seed gives other samples."""
+    if random_state is None or isinstance(random_state, (int, np.integer)):
+        random_state = np.random.RandomState(random_state)
+    x = np.asarray(x0, dtype=np.float64).reshape(-1)
+    cQ = np.linalg.cholesky(np.asarray(Q, dtype=np.float64).reshape(x.size, x.size))
+    noises = random_state.randn(T, x.size) @ cQ.T
+    states = np.empty((T + 1, x.size))
+    obs = np.empty((T, x.size))
+    states[0] = x
+    for k in range(T):
+        x = np.log(44.7) + x - np.exp(x) + noises[k]
+        states[k + 1] = x
+        obs[k] = random_state.poisson(lam * np.exp(x)).astype(np.float64)
+    return states, obs
+
+
 def lgssm_data(x0, A, H, R, Q, b, c, T, random_state=None, chol_R=None):
     """Returns (true_states (T+1, nx), observations (T, ny)), float32."""
     if random_state is None or isinstance(random_state, (int, np.integer)):

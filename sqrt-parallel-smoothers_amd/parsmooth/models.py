@@ -169,6 +169,40 @@ def make_parameters(qc, qw, r, dt, s1, s2):
     return Q, R, BearingsObservation(s1, s2), CoordinatedTurnTransition(dt)
 
 
+def population_parameters(lam, Q):
+    """Conditional moments of the stochastic Ricker map with Poisson counts
+    (notebooks/population_model.py:132-169): x' = log(44.7) + x - exp(x) + q,
+    q ~ N(0, Q); y | x ~ Poisson(lam exp(x)).  Returns, in the reference's
+    order, torch callables of the state (shape (1,)) for the transition mean,
+    transition covariance, observation mean, observation covariance,
+    transition chol and observation chol -- the pieces of the two
+    ``ConditionalMomentsModel``s."""
+    Qm = np.asarray(Q, dtype=np.float64).reshape(1, 1)
+    lam = float(lam)
+    c = lambda a, x: torch.as_tensor(a, dtype=x.dtype, device=x.device)   # noqa: E731
+
+    def mean_transition(x):
+        return float(np.log(44.7)) + x - torch.exp(x)
+
+    def cov_transition(x):
+        return c(Qm, x)
+
+    def chol_transition(x):
+        return c(np.sqrt(Qm), x)
+
+    def mean_observation(x):
+        return lam * torch.exp(x)
+
+    def cov_observation(x):
+        return (lam * torch.exp(x)).reshape(1, 1)
+
+    def chol_observation(x):
+        return torch.sqrt(lam * torch.exp(x)).reshape(1, 1)
+
+    return (mean_transition, cov_transition, mean_observation, cov_observation, chol_transition,
+            chol_observation)
+
+
 def is_device_function(f) -> bool:
     return isinstance(f, DeviceFunction)
 

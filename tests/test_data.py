@@ -1,5 +1,6 @@
 """Input generators (CPU)."""
 import numpy as np
+import pytest
 import scipy.linalg as la
 
 from parsmooth.data import bearings_data, bearings_data_batch, lgssm_data
@@ -42,3 +43,32 @@ def test_lgssm_shapes_and_determinism():
     s2, y2 = lgssm_data(np.zeros(2), A, np.eye(1, 2), np.eye(1), np.eye(2), np.zeros(2), np.zeros(1), 10, random_state=0)
     assert y1.shape == (10, 1) and s1.shape == (11, 2)
     np.testing.assert_array_equal(y1, y2)
+
+
+def test_population_data_shapes_and_dynamics():
+    """notebooks/population_model.py:172-208: T+1 states with x0 first, T
+    non-negative integer counts; without noise the states follow the Ricker map."""
+    from parsmooth.data import population_data
+    states, obs = population_data(np.array([1.0]), 50, np.array([[0.09]]), 10.0, random_state=3)
+    assert states.shape == (51, 1) and obs.shape == (50, 1)
+    assert states[0, 0] == 1.0
+    assert np.all(obs >= 0) and np.all(obs == np.round(obs))
+    quiet, _ = population_data(np.array([1.0]), 5, np.array([[1e-300]]), 10.0, random_state=0)
+    x = 1.0
+    for k in range(5):
+        x = np.log(44.7) + x - np.exp(x)
+        assert quiet[k + 1, 0] == pytest.approx(x, rel=1e-12)
+
+
+def test_population_parameters_values():
+    """The six callables of notebooks/population_model.py:132-169 at a point."""
+    import torch
+    from parsmooth.models import population_parameters
+    mt, ct, mo, co, cht, cho = population_parameters(10.0, np.array([[0.09]]))
+    x = torch.tensor([0.7], dtype=torch.float64)
+    assert float(mt(x)) == pytest.approx(np.log(44.7) + 0.7 - np.exp(0.7), rel=1e-14)
+    assert float(ct(x)) == pytest.approx(0.09) and float(cht(x)) == pytest.approx(0.3)
+    assert float(mo(x)) == pytest.approx(10 * np.exp(0.7), rel=1e-14)
+    assert float(co(x)) == pytest.approx(10 * np.exp(0.7), rel=1e-14)
+    assert float(cho(x)) == pytest.approx(np.sqrt(10 * np.exp(0.7)), rel=1e-14)
+    assert tuple(co(x).shape) == (1, 1) and tuple(cho(x).shape) == (1, 1)

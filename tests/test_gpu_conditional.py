@@ -205,3 +205,35 @@ def test_get_conditional_model_with_device_constants():
                      O.MVNSqrt(m, L))
     for u, v, name in zip(got, ref, "FCb"):
         close(npy(u), v, 1e-10, f"device-constant model {name}")
+
+
+@pytest.mark.parametrize("sqrt", [True, False])
+def test_population_helpers_match_oracle(sqrt):
+    """parsmooth.models.population_parameters on parsmooth.data.population_data
+    (notebooks/population_model.py): filter and smoother at a nominal, and three
+    IPLS iterations, against the oracle's population model."""
+    from parsmooth.data import population_data
+    from parsmooth.models import population_parameters
+    Q, lam, T = np.array([[0.09]]), 10.0, 80
+    states, ys = population_data(np.array([1.0]), T, Q, lam, random_state=4)
+    mt, ct, mo, co, cht, cho = population_parameters(lam, Q)
+    tm = ConditionalMomentsModel(mt, cht if sqrt else ct)
+    om = ConditionalMomentsModel(mo, cho if sqrt else co)
+    to = O.ConditionalMomentsModel(_PopMean(), _CondConst(np.array([[0.3 if sqrt else 0.09]])))
+    oo = O.ConditionalMomentsModel(_PoissonMean(lam), _PoissonFactor(lam, sqrt))
+    rng = np.random.RandomState(1)
+    nm, nL = states + 0.1 * rng.randn(T + 1, 1), np.full((T + 1, 1, 1), 0.5)
+    if sqrt:
+        x0d, x0o = MVNSqrt(t64([1.0]), t64([[1.0]])), O.MVNSqrt(np.array([1.0]), np.array([[1.0]]))
+        nomd, nomo = MVNSqrt(t64(nm), t64(nL)), O.MVNSqrt(nm, nL)
+    else:
+        x0d, x0o = MVNStandard(t64([1.0]), t64([[1.0]])), O.MVNStandard(np.array([1.0]), np.array([[1.0]]))
+        nomd, nomo = MVNStandard(t64(nm), t64(nL ** 2)), O.MVNStandard(nm, nL ** 2)
+    got = parsmooth.filter_smoother(t64(ys), x0d, tm, om, cubature, nomd)
+    ref = O.filter_smoother(ys, x0o, to, oo, O.cubature, nomo, True)
+    close(npy(got[0]), ref[0], 1e-9, "smoothed mean")
+    close(npy(got[1]), ref[1], 1e-9, "smoothed chol/cov")
+    crit = lambda i, *_: i < 3
+    got = parsmooth.iterated_smoothing(t64(ys), x0d, tm, om, cubature, nomd, criterion=crit)
+    ref = O.iterated_smoothing(ys, x0o, to, oo, O.cubature, nomo, criterion=crit)
+    close(npy(got[0]), ref[0], 1e-9, "IPLS mean")
