@@ -1,0 +1,79 @@
+"""The bench workload at its full size (bearings-only IEKS, T = 2^17, FP64 and
+FP32) against the C restatement of the sequential square-root filter and
+smoother (oracle/psm_oracle_c.c), which finishes one pass in about a second."""
+import numpy as np
+import pytest
+import torch
+
+pytestmark = pytest.mark.gpu
+
+if not torch.cuda.is_available():  # pragma: no cover
+    pytest.skip("needs a HIP device", allow_module_level=True)
+
+from oracle import psm_oracle_c as C  # noqa: E402
+from parsmooth import FunctionalModel, MVNSqrt  # noqa: E402
+from parsmooth.batch import IteratedSmoother  # noqa: E402
+from parsmooth.data import bearings_data  # noqa: E402
+from parsmooth.linearization import extended  # noqa: E402
+from parsmooth.models import make_parameters  # noqa: E402
+
+DEV = torch.device("cuda", 0)
+T = 1 << 17
+S1, S2 = (-1.5, 0.5), (1.0, 1.0)
+M0 = np.array([0.1, 0.2, 1.0, 0.0, 0.0])
+INIT = np.array([-1.0, -1.0, 6.0, 4.0, 2.0])
+
+
+def rel(a, b):
+    a, b = np.asarray(a, np.float64), np.asarray(b, np.float64)
+    return float(np.abs(a - b).max()) / max(1.0, float(np.abs(b).max()))
+
+
+def _problem():
+    _, _, ys = bearings_data(np.array([0.1, 0.2, 1.0, 0.0]), 0.01, 0.5, T, S1, S2, random_state=0)
+    Q, R, obs_f, trans_f = make_parameters(0.01, 0.1, 0.5, 0.01, S1, S2)
+    return np.asarray(ys, np.float64), np.linalg.cholesky(Q), np.linalg.cholesky(R), obs_f, trans_f
+
+
+def _device(dtype, n_iter):
+    ys, cQ, cR, obs_f, trans_f = _problem()
+    t = lambda a: torch.as_tensor(np.asarray(a, np.float64), device=DEV).to(dtype)   # noqa: E731
+    r = IteratedSmoother([T], FunctionalModel(trans_f, MVNSqrt(t(np.zeros(5)), t(cQ))),
+                         FunctionalModel(obs_f, MVNSqrt(t(np.zeros(2)), t(cR))), extended, dtype=dtype)
+    r.set_data(t(ys), t(M0[None]), t(np.eye(5)[None]))
+    init = (t(np.tile(INIT, (T + 1, 1))), t(np.tile(np.eye(5), (T + 1, 1, 1))))
+    m, L, ell = r.run(n_iter, init, return_loglikelihood=True)
+    return m.double().cpu().numpy(), L.double().cpu().numpy(), float(ell[0])
+
+
+def _oracle(n_iter):
+    ys, cQ, cR, _, _ = _problem()
+    return C.iterated_smoother(ys, M0, np.eye(5), np.tile(INIT, (T + 1, 1)), ("ct", 0.01), ("bearings", (S1, S2)),
+                               np.zeros(5), cQ, np.zeros(2), cR, n_iter, want_ell=True)
+
+
+def gram(X):
+    return X @ np.swapaxes(X, -1, -2)
+
+
+@pytest.mark.parametrize("n_iter", [1, 10])
+def test_full_size_fp64_matches_c_oracle(n_iter):
+    """One pass from the notebook's nominal, and the whole bench workload (10
+    iterations): smoothed means, covariances and the log-likelihood at the
+    final nominal within the FP64 bar, 1e-9 relative to max(1, |x|)."""
+    rm, rL, rell = _oracle(n_iter)
+    m, L, ell = _device(torch.float64, n_iter)
+    errs = (rel(m, rm), rel(gram(L), gram(rL)), abs(ell - rell) / max(1.0, abs(rell)))
+    print(f"FP64 n_iter {n_iter}: mean {errs[0]:.2e} cov {errs[1]:.2e} ell {errs[2]:.2e}", flush=True)
+    assert errs[0] <= 1e-9 and errs[1] <= 1e-9 and errs[2] <= 1e-9, errs
+
+
+def test_full_size_fp32_pass_matches_c_oracle():
+    """One FP32 pass at full size against the FP64 restatement (means 1e-4,
+    covariances 1e-3 -- L L^T of an FP32 factor doubles the rounding --
+    log-likelihood 1e-5; it accumulates in FP64)."""
+    rm, rL, rell = _oracle(1)
+    m, L, ell = _device(torch.float32, 1)
+    errs = (rel(m, rm), rel(gram(L), gram(rL)), abs(ell - rell) / max(1.0, abs(rell)))
+    print(f"FP32 one pass: mean {errs[0]:.2e} cov {errs[1]:.2e} ell {errs[2]:.2e}", flush=True)
+    assert errs[0] <= 1e-4 and errs[1] <= 1e-3 and errs[2] <= 1e-5, errs
