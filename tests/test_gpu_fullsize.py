@@ -14,7 +14,7 @@ from oracle import psm_oracle_c as C  # noqa: E402
 from parsmooth import FunctionalModel, MVNSqrt, MVNStandard  # noqa: E402
 from parsmooth.batch import IteratedSmoother  # noqa: E402
 from parsmooth.data import bearings_data  # noqa: E402
-from parsmooth.linearization import extended  # noqa: E402
+from parsmooth.linearization import cubature, extended  # noqa: E402
 from parsmooth.models import make_parameters  # noqa: E402
 
 DEV = torch.device("cuda", 0)
@@ -35,13 +35,13 @@ def _problem():
     return np.asarray(ys, np.float64), np.linalg.cholesky(Q), np.linalg.cholesky(R), obs_f, trans_f
 
 
-def _device(dtype, n_iter, sqrt=True):
+def _device(dtype, n_iter, sqrt=True, method=extended):
     ys, cQ, cR, obs_f, trans_f = _problem()
     t = lambda a: torch.as_tensor(np.asarray(a, np.float64), device=DEV).to(dtype)   # noqa: E731
     mvn = MVNSqrt if sqrt else MVNStandard
     q, r_ = (cQ, cR) if sqrt else (cQ @ cQ.T, cR @ cR.T)
     r = IteratedSmoother([T], FunctionalModel(trans_f, mvn(t(np.zeros(5)), t(q))),
-                         FunctionalModel(obs_f, mvn(t(np.zeros(2)), t(r_))), extended, dtype=dtype, sqrt=sqrt)
+                         FunctionalModel(obs_f, mvn(t(np.zeros(2)), t(r_))), method, dtype=dtype, sqrt=sqrt)
     r.set_data(t(ys), t(M0[None]), t(np.eye(5)[None]))
     init = (t(np.tile(INIT, (T + 1, 1))), t(np.tile(np.eye(5), (T + 1, 1, 1))))
     m, L, ell = r.run(n_iter, init, return_loglikelihood=True)
@@ -81,11 +81,12 @@ def test_full_size_fp32_pass_matches_c_oracle():
     assert errs[0] <= 1e-4 and errs[1] <= 1e-3 and errs[2] <= 1e-5, errs
 
 
-def test_full_size_covariance_form_matches_square_root_form():
+@pytest.mark.parametrize("method", [extended, cubature], ids=["extended", "cubature"])
+def test_full_size_covariance_form_matches_square_root_form(method):
     """The covariance form (MVNStandard) of one full-size FP64 pass equals the
-    square-root form's, which the test above pins to the C restatement."""
-    m_s, L_s, ell_s = _device(torch.float64, 1)
-    m_c, P_c, ell_c = _device(torch.float64, 1, sqrt=False)
+    square-root form's (pinned to the C restatement above for extended)."""
+    m_s, L_s, ell_s = _device(torch.float64, 1, method=method)
+    m_c, P_c, ell_c = _device(torch.float64, 1, sqrt=False, method=method)
     errs = (rel(m_c, m_s), rel(P_c, gram(L_s)), abs(ell_c - ell_s) / max(1.0, abs(ell_s)))
     print(f"covariance vs square-root form: mean {errs[0]:.2e} cov {errs[1]:.2e} ell {errs[2]:.2e}", flush=True)
     assert errs[0] <= 1e-9 and errs[1] <= 1e-9 and errs[2] <= 1e-9, errs
