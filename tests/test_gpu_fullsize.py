@@ -90,3 +90,47 @@ def test_full_size_covariance_form_matches_square_root_form(method):
     errs = (rel(m_c, m_s), rel(P_c, gram(L_s)), abs(ell_c - ell_s) / max(1.0, abs(ell_s)))
     print(f"covariance vs square-root form: mean {errs[0]:.2e} cov {errs[1]:.2e} ell {errs[2]:.2e}", flush=True)
     assert errs[0] <= 1e-9 and errs[1] <= 1e-9 and errs[2] <= 1e-9, errs
+
+
+def test_full_size_sequential_pass_matches_c_oracle():
+    """parallel=False at full size: one lane runs the sequential filter and
+    smoother through all 2^17 steps at the notebook's nominal; the C
+    restatement runs the same recursion (FP64 bar)."""
+    import parsmooth
+    ys, cQ, cR, obs_f, trans_f = _problem()
+    rm, rL, rell = _oracle(1)
+    t = lambda a: torch.as_tensor(np.asarray(a, np.float64), device=DEV)   # noqa: E731
+    trans = FunctionalModel(trans_f, MVNSqrt(t(np.zeros(5)), t(cQ)))
+    obs = FunctionalModel(obs_f, MVNSqrt(t(np.zeros(2)), t(cR)))
+    nom = MVNSqrt(t(np.tile(INIT, (T + 1, 1))), t(np.tile(np.eye(5), (T + 1, 1, 1))))
+    x0 = MVNSqrt(t(M0), t(np.eye(5)))
+    out = parsmooth.filter_smoother(t(ys), x0, trans, obs, extended, nom, parallel=False)
+    m, L = out[0].cpu().numpy(), out[1].cpu().numpy()
+    errs = (rel(m, rm), rel(gram(L), gram(rL)))
+    print(f"sequential pass: mean {errs[0]:.2e} cov {errs[1]:.2e}", flush=True)
+    assert errs[0] <= 1e-9 and errs[1] <= 1e-9, errs
+
+
+def test_full_size_sample_mean():
+    """64 pathwise samples of the full-size smoothing distribution (seeded,
+    filter, smoother and sampler at one nominal):
+    at 99.9 % of the states every coordinate of the sample mean lies within 4
+    standard errors of the smoothed mean, and at every state within 7."""
+    import parsmooth
+    ys, cQ, cR, obs_f, trans_f = _problem()
+    t = lambda a: torch.as_tensor(np.asarray(a, np.float64), device=DEV)   # noqa: E731
+    trans = FunctionalModel(trans_f, MVNSqrt(t(np.zeros(5)), t(cQ)))
+    obs = FunctionalModel(obs_f, MVNSqrt(t(np.zeros(2)), t(cR)))
+    x0 = MVNSqrt(t(M0), t(np.eye(5)))
+    nom = MVNSqrt(t(np.tile(INIT, (T + 1, 1))), t(np.tile(np.eye(5), (T + 1, 1, 1))))
+    filt = parsmooth.filtering(t(ys), x0, trans, obs, extended, nom)
+    smooth = parsmooth.smoothing(trans, filt, extended, nom)
+    S = 64
+    # the samples' mean is the smoother's at the sampler's own nominal: pass the same one
+    samples = parsmooth.sampling(11, S, trans, filt, extended, nom).cpu().numpy()
+    sm, sL = smooth[0].cpu().numpy(), smooth[1].cpu().numpy()
+    se = np.sqrt(np.einsum("kij,kij->ki", sL, sL) / S)           # sqrt(diag(L L^T) / S)
+    z = np.abs(samples.mean(axis=1) - sm) / np.maximum(se, 1e-300)
+    frac = float(np.mean(np.all(z <= 4.0, axis=1)))
+    print(f"sample mean: {frac:.5f} of states within 4 SE, max z {z.max():.2f}", flush=True)
+    assert frac >= 0.999 and z.max() <= 7.0, (frac, float(z.max()))
