@@ -242,6 +242,26 @@ int psm_filter_smoother(const psm_problem_t* prob, const psm_model_t* trans,
                         void* filt_m, void* filt_L, void* out_m, void* out_L, void* workspace,
                         size_t workspace_bytes, int32_t use_graph, void* stream);
 
+/* The whole fixed-count iteration of iterated smoothing as one launch
+ * sequence: n_iter filter + smoother passes, pass k at the previous pass's
+ * smoothed trajectory (pass 0 at init), then, when ell != NULL, the
+ * log-likelihood filter pass at the final nominal (ell: n_seq doubles; the
+ * filtered trajectory of that pass is left in filt_m / filt_L).  Replaces
+ * parsmooth/methods.py:54-76 with criterion i < n_iter (the fixed_point
+ * while_loop of parsmooth/_utils.py:136-146 run on the device).  The passes
+ * write out0 and out1 alternately: the result is in out0 when n_iter is odd,
+ * in out1 when it is even (out1 may be NULL for n_iter == 1); out0 must not
+ * alias init.  use_graph != 0 captures the whole sequence in one hipGraph on
+ * first use of this argument set and replays it afterwards (no host work
+ * between passes). */
+int psm_iterated_smoother_graph(const psm_problem_t* prob, const psm_model_t* trans,
+                                const psm_model_t* obs, const psm_sigma_t* sigma, const void* ys,
+                                const void* m0, const void* L0, const void* init_m,
+                                const void* init_L, int32_t n_iter, void* filt_m, void* filt_L,
+                                void* out0_m, void* out0_L, void* out1_m, void* out1_L,
+                                double* ell, void* workspace, size_t workspace_bytes,
+                                int32_t use_graph, void* stream);
+
 /* Pathwise samples of the smoothing distribution at a nominal trajectory.
  * Replaces parsmooth/_pathwise_sampler.py:13-38 (parallel sampling, reached
  * from parsmooth/methods.py:79-91) with its elements :64-127.  From the
@@ -280,7 +300,12 @@ int psm_autotune(const psm_problem_t* prob, const psm_model_t* trans, const psm_
                  void* out_L, void* workspace, size_t workspace_bytes, void* stream,
                  int32_t* best_chunk, int32_t* best_scan_k, float* best_ms);
 
-/* Drop every cached hipGraph (call before freeing buffers they reference). */
+/* Drop every cached hipGraph (call before freeing buffers they reference).
+ * The cache keeps at most 64 graphs and drops the least recently used one
+ * (after its last launch has finished) when a new one is captured.  Graphs of
+ * a caller on a stream of its own are captured on and replayed on that
+ * stream, so calls on different streams stay independent; callers on the
+ * default (legacy / per-thread) stream share one side stream. */
 void psm_clear_graphs(void);
 
 /* Kernel timing for measurement: when enabled, each kernel launched by

@@ -1,7 +1,7 @@
 // extern "C" entry points of libparsmooth_hip.so (declared in include/psm.h):
 // argument checking, dispatch to the compiled (dtype, form, nx, ny) slice,
 // the per-thread error string, the stopping-criterion reduction and the
-// hipGraph cache of psm_filter_smoother.
+// hipGraph cache of psm_filter_smoother / psm_iterated_smoother_graph.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -118,13 +118,26 @@ __global__ __launch_bounds__(1024) void k_mean_sq_diff(int64_t n, const R* __res
 }
 
 // ---------------------------------------------------------------- graph cache
+// Captured launch sequences, keyed by everything the launches depend on (dims,
+// the resolved schedule, models, every buffer address, the caller's stream).
+// A caller on a stream of its own gets its graph captured on and replayed on
+// that stream; callers on the legacy / per-thread default stream (which cannot
+// be captured) share one non-blocking side stream, ordered after and before
+// the caller's stream by two events created once.  At most kMaxGraphs graphs
+// are kept; the least recently used one is dropped (after its last launch has
+// finished) to make room.
 struct GraphEntry {
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
+  hipStream_t last = nullptr;  // stream of the last launch
+  uint64_t used = 0;           // graph-clock value of the last launch
 };
+constexpr size_t kMaxGraphs = 64;
 static std::mutex g_graph_mu;
 static std::map<std::vector<uint64_t>, GraphEntry> g_graphs;
+static uint64_t g_graph_clock = 0;
 static hipStream_t g_side_stream = nullptr;
+static hipEvent_t g_side_in = nullptr, g_side_out = nullptr;
 
 static void push(std::vector<uint64_t>& k, const void* p) { k.push_back((uint64_t)(uintptr_t)p); }
 static void push_model(std::vector<uint64_t>& k, const psm_model_t* m) {
@@ -132,6 +145,102 @@ static void push_model(std::vector<uint64_t>& k, const psm_model_t* m) {
   k.push_back((uint64_t)m->kind);
   k.push_back(((uint64_t)m->nin << 32) | (uint32_t)m->nout);
   push(k, m->params); push(k, m->q_mean); push(k, m->q_chol);
+}
+static void push_problem(std::vector<uint64_t>& k, const psm_problem_t* p, const OpsTable* t) {
+  k.push_back(((uint64_t)p->dtype << 48) | ((uint64_t)p->form << 40) | ((uint64_t)p->nx << 32) |
+              ((uint64_t)p->ny << 16));
+  k.push_back(t->plan_signature(p));
+  k.push_back((uint64_t)p->n_seq);
+  k.push_back((uint64_t)p->n_states);
+  push(k, p->state_offsets);
+  uint32_t fr;
+  memcpy(&fr, &p->fallback_ratio, sizeof(fr));
+  k.push_back((uint64_t)fr);
+  push(k, p->fallback_count);
+}
+static void push_sigma(std::vector<uint64_t>& k, const psm_sigma_t* sigma) {
+  if (sigma && sigma->npts > 0) {
+    k.push_back((uint64_t)sigma->npts);
+    push(k, sigma->xi); push(k, sigma->wm); push(k, sigma->wc);
+  } else {
+    k.push_back(0);
+  }
+}
+
+static bool default_stream(hipStream_t st) {
+  return st == nullptr || st == hipStreamLegacy || st == hipStreamPerThread;
+}
+
+static void drop_graph(GraphEntry& ge) {
+  if (ge.last) (void)hipStreamSynchronize(ge.last);
+  if (ge.exec) (void)hipGraphExecDestroy(ge.exec);
+  if (ge.graph) (void)hipGraphDestroy(ge.graph);
+}
+
+// Run `run` (the launch sequence, on the stream it is given) as a graph: captured
+// on first use of `key`, replayed afterwards.  A caller whose stream is being
+// captured already gets the launches recorded into its own graph instead.
+template <typename Fn>
+static int run_as_graph(std::vector<uint64_t>& key, hipStream_t st, Fn&& run) {
+  hipStreamCaptureStatus cap_st = hipStreamCaptureStatusNone;
+  if (!default_stream(st) && hipStreamIsCapturing(st, &cap_st) == hipSuccess &&
+      cap_st != hipStreamCaptureStatusNone)
+    return run(st);
+  push(key, st);
+  std::lock_guard<std::mutex> lock(g_graph_mu);
+  const bool side = default_stream(st);
+  if (side && g_side_stream == nullptr) {
+    if (hipStreamCreateWithFlags(&g_side_stream, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&g_side_in, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&g_side_out, hipEventDisableTiming) != hipSuccess) {
+      set_error("psm graph: side stream / event creation failed");
+      return PSM_E_HIP;
+    }
+  }
+  hipStream_t cs = side ? g_side_stream : st;
+  auto it = g_graphs.find(key);
+  if (it == g_graphs.end()) {
+    if (g_graphs.size() >= kMaxGraphs) {
+      auto lru = g_graphs.begin();
+      for (auto j = g_graphs.begin(); j != g_graphs.end(); ++j)
+        if (j->second.used < lru->second.used) lru = j;
+      drop_graph(lru->second);
+      g_graphs.erase(lru);
+    }
+    GraphEntry ge;
+    if (hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+      set_error("hipStreamBeginCapture failed");
+      return PSM_E_HIP;
+    }
+    int r = run(cs);
+    hipError_t e = hipStreamEndCapture(cs, &ge.graph);
+    if (r) { if (ge.graph) (void)hipGraphDestroy(ge.graph); return r; }
+    if (e != hipSuccess) { set_error(std::string("hipStreamEndCapture: ") + hipGetErrorString(e)); return PSM_E_HIP; }
+    e = hipGraphInstantiate(&ge.exec, ge.graph, nullptr, nullptr, 0);
+    if (e != hipSuccess) {
+      (void)hipGraphDestroy(ge.graph);
+      set_error(std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+      return PSM_E_HIP;
+    }
+    it = g_graphs.emplace(key, ge).first;
+  }
+  hipError_t e;
+  if (side) {
+    (void)hipEventRecord(g_side_in, st);
+    (void)hipStreamWaitEvent(cs, g_side_in, 0);
+    e = hipGraphLaunch(it->second.exec, cs);
+    (void)hipEventRecord(g_side_out, cs);
+    (void)hipStreamWaitEvent(st, g_side_out, 0);
+  } else {
+    e = hipGraphLaunch(it->second.exec, cs);
+  }
+  it->second.last = cs;
+  it->second.used = ++g_graph_clock;
+  if (e != hipSuccess) {
+    set_error(std::string("hipGraphLaunch: ") + hipGetErrorString(e));
+    return PSM_E_HIP;
+  }
+  return PSM_OK;
 }
 
 // ---------------------------------------------------------------- timing
@@ -289,78 +398,71 @@ int psm_filter_smoother(const psm_problem_t* p, const psm_model_t* trans, const 
   };
   hipStream_t st = (hipStream_t)stream;
   if (!use_graph) return run(st);
-
-  std::vector<uint64_t> key;
-  key.push_back(((uint64_t)p->dtype << 48) | ((uint64_t)p->form << 40) | ((uint64_t)p->nx << 32) |
-                ((uint64_t)p->ny << 16) | (uint64_t)(uint16_t)p->chunk);
-  key.push_back((uint64_t)(uint32_t)p->scan_k);
-  key.push_back((uint64_t)p->n_seq);
-  key.push_back((uint64_t)p->n_states);
-  push(key, p->state_offsets);
-  {
-    uint32_t fr;
-    memcpy(&fr, &p->fallback_ratio, sizeof(fr));
-    key.push_back((uint64_t)fr);
-  }
-  push(key, p->fallback_count);
+  std::vector<uint64_t> key = {0x46534d4f4f5448ull};  // one filter-smoother pass
+  push_problem(key, p, t);
   push_model(key, trans);
   push_model(key, obs);
-  if (sigma && sigma->npts > 0) {
-    key.push_back((uint64_t)sigma->npts);
-    push(key, sigma->xi); push(key, sigma->wm); push(key, sigma->wc);
-  } else {
-    key.push_back(0);
-  }
+  push_sigma(key, sigma);
   for (const void* q : {ys, m0, L0, nom_m, nom_L, (const void*)filt_m, (const void*)filt_L,
                         (const void*)out_m, (const void*)out_L, (const void*)ws})
     push(key, q);
   key.push_back((uint64_t)wsb);
-  push(key, stream);
+  return run_as_graph(key, st, run);
+}
 
-  std::lock_guard<std::mutex> lock(g_graph_mu);
-  // Capture and replay on a non-default stream; order it after/before the
-  // caller's stream with events so the legacy default stream works too.
-  if (g_side_stream == nullptr) {
-    if (hipStreamCreateWithFlags(&g_side_stream, hipStreamNonBlocking) != hipSuccess) {
-      set_error("hipStreamCreate failed");
-      return PSM_E_HIP;
-    }
+int psm_iterated_smoother_graph(const psm_problem_t* p, const psm_model_t* trans,
+                                const psm_model_t* obs, const psm_sigma_t* sigma, const void* ys,
+                                const void* m0, const void* L0, const void* init_m,
+                                const void* init_L, int32_t n_iter, void* filt_m, void* filt_L,
+                                void* out0_m, void* out0_L, void* out1_m, void* out1_L,
+                                double* ell, void* ws, size_t wsb, int32_t use_graph,
+                                void* stream) {
+  int rc = check_problem(p);
+  if (rc) return rc;
+  if ((rc = check_model(trans, p->nx, p->nx))) return rc;
+  if ((rc = check_model(obs, p->nx, p->ny))) return rc;
+  if (n_iter < 1) { set_error("psm_iterated_smoother_graph: n_iter must be >= 1"); return PSM_E_ARG; }
+  if (!ys || !m0 || !L0 || !init_m || !init_L || !filt_m || !filt_L || !out0_m || !out0_L ||
+      (n_iter > 1 && (!out1_m || !out1_L))) {
+    set_error("psm_iterated_smoother_graph: null array");
+    return PSM_E_ARG;
   }
-  hipStream_t cs = g_side_stream;
-  auto it = g_graphs.find(key);
-  if (it == g_graphs.end()) {
-    GraphEntry ge;
-    if (hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal) != hipSuccess) {
-      set_error("hipStreamBeginCapture failed");
-      return PSM_E_HIP;
-    }
-    int r = run(cs);
-    hipError_t e = hipStreamEndCapture(cs, &ge.graph);
-    if (r) { if (ge.graph) (void)hipGraphDestroy(ge.graph); return r; }
-    if (e != hipSuccess) { set_error(std::string("hipStreamEndCapture: ") + hipGetErrorString(e)); return PSM_E_HIP; }
-    e = hipGraphInstantiate(&ge.exec, ge.graph, nullptr, nullptr, 0);
-    if (e != hipSuccess) {
-      (void)hipGraphDestroy(ge.graph);
-      set_error(std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
-      return PSM_E_HIP;
-    }
-    it = g_graphs.emplace(key, ge).first;
+  if (out0_m == init_m || out0_L == init_L) {
+    set_error("psm_iterated_smoother_graph: out0 must not alias the initial nominal");
+    return PSM_E_ARG;
   }
-  hipEvent_t ev_in, ev_out;
-  (void)hipEventCreateWithFlags(&ev_in, hipEventDisableTiming);
-  (void)hipEventCreateWithFlags(&ev_out, hipEventDisableTiming);
-  (void)hipEventRecord(ev_in, st);
-  (void)hipStreamWaitEvent(cs, ev_in, 0);
-  hipError_t e = hipGraphLaunch(it->second.exec, cs);
-  (void)hipEventRecord(ev_out, cs);
-  (void)hipStreamWaitEvent(st, ev_out, 0);
-  (void)hipEventDestroy(ev_in);
-  (void)hipEventDestroy(ev_out);
-  if (e != hipSuccess) {
-    set_error(std::string("hipGraphLaunch: ") + hipGetErrorString(e));
-    return PSM_E_HIP;
-  }
-  return PSM_OK;
+  const OpsTable* t = need(p->dtype, p->form, p->nx, p->ny);
+  if (!t) return PSM_E_UNSUPPORTED;
+  void* bm[2] = {out0_m, out1_m};
+  void* bL[2] = {out0_L, out1_L};
+  auto run = [&](hipStream_t s) -> int {
+    const void* nm = init_m;
+    const void* nL = init_L;
+    for (int k = 0; k < n_iter; ++k) {
+      int r = t->filter(p, trans, obs, sigma, ys, m0, L0, nm, nL, filt_m, filt_L, nullptr, ws, wsb, s, 0);
+      if (r) return r;
+      r = t->smoother(p, trans, sigma, filt_m, filt_L, nm, nL, bm[k & 1], bL[k & 1], ws, wsb, s, 1);
+      if (r) return r;
+      nm = bm[k & 1];
+      nL = bL[k & 1];
+    }
+    if (ell != nullptr)
+      return t->filter(p, trans, obs, sigma, ys, m0, L0, nm, nL, filt_m, filt_L, ell, ws, wsb, s, 0);
+    return PSM_OK;
+  };
+  hipStream_t st = (hipStream_t)stream;
+  if (!use_graph) return run(st);
+  std::vector<uint64_t> key = {0x49544552415445ull, (uint64_t)n_iter};  // the whole iteration
+  push_problem(key, p, t);
+  push_model(key, trans);
+  push_model(key, obs);
+  push_sigma(key, sigma);
+  for (const void* q : {ys, m0, L0, init_m, init_L, (const void*)filt_m, (const void*)filt_L,
+                        (const void*)out0_m, (const void*)out0_L, (const void*)out1_m,
+                        (const void*)out1_L, (const void*)ell, (const void*)ws})
+    push(key, q);
+  key.push_back((uint64_t)wsb);
+  return run_as_graph(key, st, run);
 }
 
 size_t psm_sample_workspace_bytes(const psm_problem_t* p, int32_t n_samples) {
@@ -636,10 +738,7 @@ int psm_autotune(const psm_problem_t* p, const psm_model_t* trans, const psm_mod
 
 void psm_clear_graphs(void) {
   std::lock_guard<std::mutex> lock(g_graph_mu);
-  for (auto& kv : g_graphs) {
-    if (kv.second.exec) (void)hipGraphExecDestroy(kv.second.exec);
-    if (kv.second.graph) (void)hipGraphDestroy(kv.second.graph);
-  }
+  for (auto& kv : g_graphs) drop_graph(kv.second);
   g_graphs.clear();
 }
 

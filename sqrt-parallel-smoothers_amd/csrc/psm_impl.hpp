@@ -197,6 +197,11 @@ struct OpsTable {
                     const void*, void*, void*, double*, hipStream_t);
   int (*seq_smoother)(const psm_problem_t*, const psm_model_t*, const psm_sigma_t*, const void*,
                       const void*, const void*, const void*, void*, void*, hipStream_t);
+  // the launch schedule a problem resolves to (environment defaults applied),
+  // packed: chunk | level-0 fan-in << 16 | levels << 24 | team << 32 |
+  // level-0 workgroup size << 33; a captured launch sequence is valid only for
+  // the same signature
+  uint64_t (*plan_signature)(const psm_problem_t*);
 };
 
 template <typename R, int NX, int NY, bool SQRT>
@@ -232,6 +237,17 @@ struct Impl {
   }
 
   static size_t workspace_bytes(const psm_problem_t* p) { return plan(p).total; }
+
+  static uint64_t plan_signature(const psm_problem_t* p) {
+    const Plan pl = plan(p);
+    uint64_t sig = (uint64_t)(uint32_t)pl.chunk & 0xffffu;
+    sig |= (uint64_t)(pl.lv[0].K & 0xff) << 16;
+    sig |= (uint64_t)(pl.nlevels & 0xff) << 24;
+    sig |= (uint64_t)(pl.team ? 1 : 0) << 32;
+    sig |= (uint64_t)(pl.lv[0].wg & 0xffff) << 33;
+    for (int l = 1; l < pl.nlevels; ++l) sig = sig * 1099511628211ull + (uint64_t)pl.lv[l].K;
+    return sig;
+  }
 
   template <int NIN, int NOUT>
   static int linearize_points(const psm_model_t* md, const psm_sigma_t* sg, int64_t n,
@@ -600,7 +616,8 @@ struct Impl {
     static const OpsTable t = {&linearize_trans, &linearize_obs, &filter_elements,
                                &filter_combine,  &smoother_elements, &smoother_combine,
                                &workspace_bytes, &filter,          &smoother,
-                               &sample_workspace_bytes, &sample, &seq_filter, &seq_smoother};
+                               &sample_workspace_bytes, &sample, &seq_filter, &seq_smoother,
+                               &plan_signature};
     return &t;
   }
 };

@@ -28,9 +28,8 @@ class ConditionalMomentsModel(NamedTuple):
 
 def are_inputs_compatible(*y):
     """Raise TypeError unless some consecutive pair of arguments shares its type
-    (the reference's rule, parsmooth/_base.py:25-30)."""
+    (the reference's rule, parsmooth/_base.py:25-30); with fewer than two
+    arguments there is no such pair, so it raises too."""
     types = [type(v) for v in y]
-    if len(types) < 2:
-        return
     if not any(a == b for a, b in zip(types[:-1], types[1:])):
         raise TypeError(f"All inputs should have the same type. {y} was given")

@@ -39,6 +39,7 @@ EXPORTS = (
     "psm_filter", "psm_smoother", "psm_filter_smoother", "psm_mean_sq_diff", "psm_clear_graphs",
     "psm_timing_enable", "psm_timing_report", "psm_chunk_size", "psm_autotune",
     "psm_sample_workspace_bytes", "psm_sample", "psm_seq_filter", "psm_seq_smoother",
+    "psm_iterated_smoother_graph",
 )
 
 
@@ -86,6 +87,9 @@ _SIGS = {
     "psm_filter_smoother": (ctypes.c_int, [ctypes.POINTER(PsmProblem), ctypes.POINTER(PsmModel),
                                            ctypes.POINTER(PsmModel), ctypes.POINTER(PsmSigma)]
                             + [_P] * 9 + [_P, ctypes.c_size_t, _I32, _P]),
+    "psm_iterated_smoother_graph": (ctypes.c_int, [ctypes.POINTER(PsmProblem), ctypes.POINTER(PsmModel),
+                                                   ctypes.POINTER(PsmModel), ctypes.POINTER(PsmSigma)]
+                                    + [_P] * 5 + [_I32] + [_P] * 7 + [_P, ctypes.c_size_t, _I32, _P]),
     "psm_mean_sq_diff": (ctypes.c_int, [_I32, _I64, _P, _P, _P, _P]),
     "psm_clear_graphs": (None, []),
     "psm_timing_enable": (None, [_I32]),

@@ -9,6 +9,7 @@ runs reuse them and each pass can be replayed from a captured hipGraph.
 """
 from __future__ import annotations
 
+import ctypes
 from typing import Callable, Optional, Sequence
 
 import numpy as np
@@ -155,21 +156,40 @@ class IteratedSmoother:
         one pass at the default nominal when ``init_nominal`` is None, then
         ``n_iter`` passes.  Returns (mean, chol) of all states (views of internal
         buffers, valid until the next run) and optionally the per-sequence
-        log-likelihood at the final nominal trajectory."""
+        log-likelihood at the final nominal trajectory.
+
+        Parallel passes over device-evaluated models run as one launch
+        sequence (``psm_iterated_smoother_graph``): every pass and the final
+        log-likelihood pass are enqueued without host work in between, and with
+        ``use_graph`` the whole sequence is one hipGraph, captured on the first
+        run and replayed on later runs with the same buffers."""
         if self.ys is None:
             raise RuntimeError("call set_data first")
         if init_nominal is None:
-            cur = self.pass_once(self._default, self._bufs[0], use_graph)
-            k = 1
+            start, n = self._default, int(n_iter) + 1
         else:
             self._init[0].copy_(to_dev(init_nominal[0], self.dtype, self.dev).reshape(self._init[0].shape))
             self._init[1].copy_(to_dev(init_nominal[1], self.dtype, self.dev).reshape(self._init[1].shape))
+            start, n = self._init, int(n_iter)
+        if n == 0:
             cur = self._init
-            k = 0
-        for _ in range(int(n_iter)):
-            nxt = self._bufs[k % 2]
-            cur = self.pass_once(cur, nxt, use_graph)
-            k += 1
+        elif self.sequential or self.mods.tabulated:
+            cur = start
+            for k in range(n):
+                cur = self.pass_once(cur, self._bufs[k % 2], use_graph)
+        else:
+            ell = torch.empty(self.prob.n_seq, dtype=torch.float64, device=self.dev) \
+                if return_loglikelihood else None
+            rc = N.lib().psm_iterated_smoother_graph(
+                ctypes.byref(self.prob.struct), ctypes.byref(self.mods.trans), ctypes.byref(self.mods.obs),
+                self.mods.sigma_ref(), N.ptr(self.ys), N.ptr(self.m0), N.ptr(self.L0), N.ptr(start[0]),
+                N.ptr(start[1]), n, N.ptr(self._filt[0]), N.ptr(self._filt[1]), N.ptr(self._bufs[0][0]),
+                N.ptr(self._bufs[0][1]), N.ptr(self._bufs[1][0]), N.ptr(self._bufs[1][1]), N.ptr(ell),
+                N.ptr(self.prob.workspace), self.prob.ws_bytes, int(bool(use_graph)), N.stream_ptr())
+            N.check(rc, "psm_iterated_smoother_graph")
+            cur = self._bufs[(n - 1) % 2]
+            if return_loglikelihood:
+                return cur[0], cur[1], ell
         if return_loglikelihood:
             run = seq_filter_pass if self.sequential else filter_pass
             _, _, ell = run(self.prob, self.mods, self.ys, self.m0, self.L0, cur[0], cur[1], True,

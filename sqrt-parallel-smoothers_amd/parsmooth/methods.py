@@ -3,16 +3,23 @@
 ``parallel=True`` (the default) runs the associative-scan formulation;
 ``parallel=False`` runs the sequential recursion, one device lane per
 sequence, which without a nominal trajectory linearizes at the running
-estimate as the reference's sequential methods do.  The fixed-point loop of
-``iterated_smoothing`` runs here on the host, exactly like
+estimate as the reference's sequential methods do.
+
+The fixed-point loop of ``iterated_smoothing`` follows
 ``parsmooth/_utils.py:136-146``: the carry starts at ``(1, x0, f(x0))`` and
-``f`` is applied again while ``criterion(i, previous, current)`` holds.  Each
-application of ``f`` is one call of ``psm_filter_smoother`` (filter, then
-smoother, at the same nominal trajectory), or of ``psm_seq_filter`` and
-``psm_seq_smoother``.
+``f`` (filter, then smoother, at the same nominal trajectory) is applied
+again while ``criterion(i, previous, current)`` holds.  A criterion that
+depends on the iteration count only (``lambda i, *_: i < n``, or
+``FixedIterations(n)``) fixes the number of passes in advance; then, for the
+parallel passes over models the kernels evaluate, the whole loop and the
+final log-likelihood pass run as one hipGraph (``psm_iterated_smoother_graph``)
+that is captured once per problem shape and replayed by later calls.  Any other
+criterion runs on the host between passes, each pass one call of
+``psm_filter_smoother`` (or of ``psm_seq_filter`` and ``psm_seq_smoother``).
 """
 from __future__ import annotations
 
+from collections import OrderedDict
 from typing import Callable, Optional, Union
 
 import torch
@@ -52,6 +59,62 @@ def filter_smoother(observations, x0, transition_model, observation_model, linea
                     parallel)
     nom = it.nominal(nominal_trajectory)
     return it.step(nom)
+
+
+class FixedIterations:
+    """Criterion of exactly ``n`` fixed-point applications: ``i < n``."""
+
+    def __init__(self, n: int):
+        self.n = int(n)
+
+    def __call__(self, i, *_):
+        return i < self.n
+
+
+class _Opaque:
+    """Stands in for a trajectory while a criterion is probed: any use raises."""
+
+    def __getattr__(self, name):
+        raise _NotFixed
+
+    def __getitem__(self, key):
+        raise _NotFixed
+
+    def __iter__(self):
+        raise _NotFixed
+
+    def __array__(self, *a, **k):
+        raise _NotFixed
+
+
+class _NotFixed(Exception):
+    pass
+
+
+_PROBE_LIMIT = 1 << 16
+
+
+def _fixed_count(criterion) -> Optional[int]:
+    """The number of fixed-point applications a criterion allows when it
+    depends on the iteration count only, else None.  The criterion is called
+    with i = 1, 2, ... and stand-in trajectories that raise on any use; it is
+    a count criterion when it returns a bool each time and first returns False
+    at some i <= 65536 (that i is the number of applications)."""
+    if isinstance(criterion, FixedIterations):
+        return max(1, criterion.n)
+    if criterion is _default_criterion:
+        return None
+    opaque = _Opaque()
+    try:
+        for i in range(1, _PROBE_LIMIT + 1):
+            r = criterion(i, opaque, opaque)
+            if not isinstance(r, bool):
+                return None
+            if not r:
+                return i
+    except Exception:
+        return None
+    return None
 
 
 def _default_criterion(_i, nominal_traj_prev, curr_nominal_traj):
@@ -117,6 +180,12 @@ def iterated_smoothing(observations, x0: Union[MVNSqrt, MVNStandard], transition
                        init_nominal_trajectory=None, parallel: bool = True,
                        criterion: Callable = _default_criterion, return_loglikelihood: bool = False):
     """IEKS (extended) / IPLS (sigma-point) smoothing (methods.py:54-76)."""
+    count = _fixed_count(criterion) if parallel else None
+    if count is not None:
+        out = _whole_loop(observations, x0, transition_model, observation_model, linearization_method,
+                          init_nominal_trajectory, count, return_loglikelihood)
+        if out is not None:
+            return out
     it = _Iteration(observations, x0, transition_model, observation_model, linearization_method,
                     parallel)
     if init_nominal_trajectory is None:
@@ -129,3 +198,59 @@ def iterated_smoothing(observations, x0: Union[MVNSqrt, MVNStandard], transition
     if return_loglikelihood:
         return cur, it.loglikelihood(tuple(cur))
     return cur
+
+
+# Batch runners of recent problem shapes (device buffers and the captured
+# whole-loop graphs that reference them), so that repeated fixed-count calls
+# replay one graph.  clear_caches() releases them.
+_RUNNERS: "OrderedDict[tuple, object]" = OrderedDict()
+_MAX_RUNNERS = 2
+
+
+def clear_caches():
+    """Release the buffers of cached runners and every captured hipGraph."""
+    _RUNNERS.clear()
+    N.lib().psm_clear_graphs()
+
+
+def _whole_loop(observations, x0, transition_model, observation_model, linearization_method,
+                init_nominal_trajectory, count: int, return_loglikelihood: bool):
+    """Fixed-count iterated smoothing as one launch sequence; None when the
+    models are evaluated by torch (their tables are refilled between passes)."""
+    from .batch import IteratedSmoother
+    from .models import DeviceFunction
+    from ._base import FunctionalModel
+    if not all(isinstance(m, FunctionalModel) and isinstance(m[0], DeviceFunction)
+               for m in (transition_model, observation_model)):
+        return None
+    dtype = compute_dtype(x0)
+    dev = N.device()
+    sqrt = isinstance(x0, MVNSqrt)
+    ys = to_dev(observations, dtype, dev)
+    T = ys.shape[0]
+    ys = ys.reshape(T, -1)
+    m0 = to_dev(x0[0], dtype, dev)
+    nx, ny = m0.shape[-1], ys.shape[1]
+    method = resolve(linearization_method)
+    key = (T, nx, ny, dtype, sqrt, method.name, tuple(sorted(method.kwargs.items())))
+    runner = _RUNNERS.pop(key, None)
+    if runner is None:
+        runner = IteratedSmoother([T], transition_model, observation_model, method, dtype=dtype, sqrt=sqrt,
+                                  nx=nx, ny=ny)
+    else:
+        runner.mods = Models(runner.prob, transition_model, observation_model, method)
+    _RUNNERS[key] = runner
+    while len(_RUNNERS) > _MAX_RUNNERS:
+        _RUNNERS.popitem(last=False)
+    runner.set_data(ys, m0.reshape(1, nx), to_dev(x0[1], dtype, dev).reshape(1, nx, nx))
+    init = None
+    if init_nominal_trajectory is not None:
+        are_inputs_compatible(x0, init_nominal_trajectory)
+        init = (init_nominal_trajectory[0], init_nominal_trajectory[1])
+    # IteratedSmoother.run adds the pass at the default nominal when init is None
+    out = runner.run(count, init, use_graph=True, return_loglikelihood=return_loglikelihood)
+    cls = type(x0)
+    res = cls(out[0].clone(), out[1].clone())
+    if return_loglikelihood:
+        return res, out[2][0].clone()
+    return res

@@ -155,3 +155,32 @@ def test_sequential_entry_points_check_arguments():
     pd = N.PsmProblem(N.PSM_F64_TANGENT, N.FORM_SQRT, 5, 2, 1, 0, 11, 1)
     rc = lib.psm_seq_smoother(ctypes.byref(pd), ctypes.byref(lin), None, one, one, one, one, one, one, None)
     assert rc == -1 and b"PSM_F32 or PSM_F64" in lib.psm_last_error()
+
+
+def test_are_inputs_compatible_follows_the_reference():
+    """parsmooth/_base.py:25-30: TypeError unless some consecutive pair of
+    arguments shares its type -- so also with fewer than two arguments."""
+    from parsmooth import MVNSqrt, MVNStandard
+    from parsmooth._base import are_inputs_compatible
+    a, b = MVNSqrt(1.0, 2.0), MVNStandard(1.0, 2.0)
+    are_inputs_compatible(a, a)
+    are_inputs_compatible(a, b, b)
+    for args in ((), (a,), (a, b), (b, a, b)):
+        with pytest.raises(TypeError):
+            are_inputs_compatible(*args)
+
+
+def test_fixed_count_criterion_probe():
+    """iterated_smoothing recognises criteria that depend on the iteration count
+    only (run as one launch sequence) and leaves every other criterion to the
+    host loop."""
+    from parsmooth.methods import FixedIterations, _default_criterion, _fixed_count
+    assert _fixed_count(lambda i, *_: i < 10) == 10
+    assert _fixed_count(lambda i, prev, cur: i < 3) == 3
+    assert _fixed_count(FixedIterations(7)) == 7
+    assert _fixed_count(lambda i, *_: False) == 1
+    assert _fixed_count(_default_criterion) is None
+    assert _fixed_count(lambda i, prev, cur: i < 5 and prev.mean is not None) is None
+    assert _fixed_count(lambda i, prev, cur: bool((prev[0] - cur[0]).sum() > 0)) is None
+    assert _fixed_count(lambda i, *_: True) is None          # never stops within the probe limit
+    assert _fixed_count(lambda i, *_: 1 if i < 3 else 0) is None   # not a bool

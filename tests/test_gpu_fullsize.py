@@ -12,6 +12,7 @@ if not torch.cuda.is_available():  # pragma: no cover
 
 from oracle import psm_oracle_c as C  # noqa: E402
 from parsmooth import FunctionalModel, MVNSqrt, MVNStandard  # noqa: E402
+from parsmooth import _native as N  # noqa: E402
 from parsmooth.batch import IteratedSmoother  # noqa: E402
 from parsmooth.data import bearings_data  # noqa: E402
 from parsmooth.linearization import cubature, extended  # noqa: E402
@@ -35,7 +36,12 @@ def _problem():
     return np.asarray(ys, np.float64), np.linalg.cholesky(Q), np.linalg.cholesky(R), obs_f, trans_f
 
 
-def _device(dtype, n_iter, sqrt=True, method=extended):
+# the schedule the headline bench line runs (autotuned on MI355X): chunk 5,
+# scan fan-in 2, team-cooperative upper levels, half-wave level 0
+HEADLINE_SCHEDULE = (5, 2 | N.SCAN_TEAM | N.SCAN_HALF)
+
+
+def _device(dtype, n_iter, sqrt=True, method=extended, schedule=None):
     ys, cQ, cR, obs_f, trans_f = _problem()
     t = lambda a: torch.as_tensor(np.asarray(a, np.float64), device=DEV).to(dtype)   # noqa: E731
     mvn = MVNSqrt if sqrt else MVNStandard
@@ -43,6 +49,9 @@ def _device(dtype, n_iter, sqrt=True, method=extended):
     r = IteratedSmoother([T], FunctionalModel(trans_f, mvn(t(np.zeros(5)), t(q))),
                          FunctionalModel(obs_f, mvn(t(np.zeros(2)), t(r_))), method, dtype=dtype, sqrt=sqrt)
     r.set_data(t(ys), t(M0[None]), t(np.eye(5)[None]))
+    if schedule is not None:
+        r.prob.set_schedule(*schedule)
+        assert r.chunk_size() == schedule[0]
     init = (t(np.tile(INIT, (T + 1, 1))), t(np.tile(np.eye(5), (T + 1, 1, 1))))
     m, L, ell = r.run(n_iter, init, return_loglikelihood=True)
     return m.double().cpu().numpy(), L.double().cpu().numpy(), float(ell[0])
@@ -58,15 +67,18 @@ def gram(X):
     return X @ np.swapaxes(X, -1, -2)
 
 
-@pytest.mark.parametrize("n_iter", [1, 10])
-def test_full_size_fp64_matches_c_oracle(n_iter):
+@pytest.mark.parametrize("n_iter,schedule", [(1, None), (10, None), (10, HEADLINE_SCHEDULE)],
+                         ids=["1-default", "10-default", "10-headline"])
+def test_full_size_fp64_matches_c_oracle(n_iter, schedule):
     """One pass from the notebook's nominal, and the whole bench workload (10
-    iterations): smoothed means, covariances and the log-likelihood at the
-    final nominal within the FP64 bar, 1e-9 relative to max(1, |x|)."""
+    iterations, with the default schedule and with the headline's autotuned
+    one): smoothed means, covariances and the log-likelihood at the final
+    nominal within the FP64 bar, 1e-9 relative to max(1, |x|)."""
     rm, rL, rell = _oracle(n_iter)
-    m, L, ell = _device(torch.float64, n_iter)
+    m, L, ell = _device(torch.float64, n_iter, schedule=schedule)
     errs = (rel(m, rm), rel(gram(L), gram(rL)), abs(ell - rell) / max(1.0, abs(rell)))
-    print(f"FP64 n_iter {n_iter}: mean {errs[0]:.2e} cov {errs[1]:.2e} ell {errs[2]:.2e}", flush=True)
+    print(f"FP64 n_iter {n_iter} schedule {schedule}: mean {errs[0]:.2e} cov {errs[1]:.2e} ell {errs[2]:.2e}",
+          flush=True)
     assert errs[0] <= 1e-9 and errs[1] <= 1e-9 and errs[2] <= 1e-9, errs
 
 

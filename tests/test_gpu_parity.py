@@ -552,7 +552,7 @@ def test_every_schedule_matches_oracle(chunk):
     top16 = 16 << N.SCAN_TOP_SHIFT  # top level may fold up to 16 items per lane
     half = N.SCAN_HALF
     for k in (2, 4, 8, 2 | N.SCAN_TEAM, 8 | N.SCAN_TEAM, 2 | N.SCAN_TEAM | top16, 4 | top16,
-              2 | half, 4 | N.SCAN_TEAM | half):
+              2 | half, 4 | N.SCAN_TEAM | half, 2 | N.SCAN_TEAM | half):
         r, obs, m0, nom = _schedule_runner(lengths, 3, chunk, k)
         assert r.chunk_size() == chunk
         out = r.prob.new_traj()

@@ -378,6 +378,85 @@ def test_c_restatement_linear_closed_form():
     assert e == pytest.approx(ell, rel=1e-9)
 
 
+def _bearings_c_case(T, seed):
+    from parsmooth.data import bearings_data
+    _, _, ys = bearings_data(np.array([0.1, 0.2, 1.0, 0.0]), 0.01, 0.5, T, (-1.5, 0.5), (1.0, 1.0),
+                             random_state=seed)
+    Q, R = O.bearings_parameters(0.01, 0.1, 0.5, 0.01)
+    return np.asarray(ys, np.float64), np.linalg.cholesky(Q), np.linalg.cholesky(R)
+
+
+@pytest.mark.parametrize("table", ["cubature", "unscented", "gauss_hermite"])
+def test_c_restatement_sigma_points_match_numpy_oracle(table):
+    """The C sigma-point regression (a full-size checker of cubature IPLS) against
+    the numpy restatement of parsmooth/linearization/_sigma_points.py:63-100:
+    three iterations of the bearings-only IPLS from a nominal with a non-identity
+    chol, smoothed trajectory and log-likelihood."""
+    C = _c_oracle()
+    T = 60
+    ys, cQ, cR = _bearings_c_case(T, 3)
+    tab = {"cubature": O.cubature_table, "unscented": O.unscented_table,
+           "gauss_hermite": O.gauss_hermite_table}[table](5)
+    meth = {"cubature": O.cubature, "unscented": O.unscented, "gauss_hermite": O.gauss_hermite}[table]
+    m0 = np.array([0.1, 0.2, 1.0, 0.0, 0.0])
+    rng = np.random.RandomState(1)
+    init_m = m0 + 0.05 * rng.randn(T + 1, 5)
+    init_L = np.tile(np.diag([0.3, 0.4, 0.5, 0.6, 0.2]), (T + 1, 1, 1))
+    to = O.FunctionalModel(O.CoordinatedTurn(0.01), O.MVNSqrt(np.zeros(5), cQ))
+    oo = O.FunctionalModel(O.Bearings((-1.5, 0.5), (1.0, 1.0)), O.MVNSqrt(np.zeros(2), cR))
+    ref, ell = O.iterated_smoothing(ys, O.MVNSqrt(m0, np.eye(5)), to, oo, meth, O.MVNSqrt(init_m, init_L), True,
+                                    criterion=lambda i, *_: i < 3, return_loglikelihood=True)
+    m, L, e, _ = C.iterated_smoother_batch([T], ys, m0[None], np.eye(5)[None], init_m, init_L, ("ct", 0.01),
+                                           ("bearings", ((-1.5, 0.5), (1.0, 1.0))), np.zeros(5), cQ, np.zeros(2),
+                                           cR, 3, sigma=(tab.xi, tab.wm, tab.wc), want_ell=True)
+    np.testing.assert_allclose(m, ref.mean, atol=1e-9)
+    np.testing.assert_allclose(L @ np.swapaxes(L, 1, 2), ref.chol @ np.swapaxes(ref.chol, 1, 2), atol=1e-9)
+    assert e[0] == pytest.approx(ell, rel=1e-10)
+
+
+def test_c_restatement_cubature_is_exact_on_linear_models():
+    """Every sigma-point rule reproduces mean and covariance, so on a linear
+    model the regression returns the model itself: cubature IPLS equals the
+    dense joint-Gaussian posterior (a dropped weight, a wrong point scale or a
+    missed downdate fails this)."""
+    C = _c_oracle()
+    x0, cx0, (F, Q, b, H, R, c), ys, _, _ = _lin_problem(3, 2, 0, 6)
+    (_, _), (sm, sP), ell = O.joint_gaussian_posterior(ys, x0.mean, x0.cov, F, Q, b, H, R, c)
+    tab = O.cubature_table(3)
+    rng = np.random.RandomState(2)
+    m, L, e, _ = C.iterated_smoother_batch([6], ys, cx0.mean[None], cx0.chol[None], rng.randn(7, 3),
+                                           np.tile(2.0 * np.eye(3), (7, 1, 1)), ("linear", F), ("linear", H), b,
+                                           np.linalg.cholesky(Q), c, np.linalg.cholesky(R), 1,
+                                           sigma=(tab.xi, tab.wm, tab.wc), want_ell=True)
+    np.testing.assert_allclose(m, sm, atol=1e-8)
+    np.testing.assert_allclose(L @ np.swapaxes(L, 1, 2), sP, atol=1e-8)
+    assert e[0] == pytest.approx(ell, rel=1e-9)
+
+
+def test_c_restatement_batch_equals_single_sequences():
+    """A ragged batch (three bearings sequences of lengths 40, 7, 25, cubature,
+    two iterations, several threads) equals each sequence run alone, bit for
+    bit: the batch offsets of states, observations and priors are right."""
+    C = _c_oracle()
+    lengths = [40, 7, 25]
+    tab = O.cubature_table(5)
+    cases = [_bearings_c_case(T, 10 + k) for k, T in enumerate(lengths)]
+    cQ, cR = cases[0][1], cases[0][2]
+    rng = np.random.RandomState(5)
+    m0 = np.array([0.1, 0.2, 1.0, 0.0, 0.0]) + 0.1 * rng.randn(3, 5)
+    init = [np.tile(m0[k], (T + 1, 1)) for k, T in enumerate(lengths)]
+    args = (("ct", 0.01), ("bearings", ((-1.5, 0.5), (1.0, 1.0))), np.zeros(5), cQ, np.zeros(2), cR, 2)
+    bm, bL, be, used = C.iterated_smoother_batch(lengths, np.concatenate([c[0] for c in cases]), m0,
+                                                 np.tile(np.eye(5), (3, 1, 1)), np.concatenate(init), None, *args,
+                                                 sigma=(tab.xi, tab.wm, tab.wc), want_ell=True, threads=3)
+    off = np.concatenate([[0], np.cumsum(np.asarray(lengths) + 1)])
+    for k, T in enumerate(lengths):
+        m, L, e, _ = C.iterated_smoother_batch([T], cases[k][0], m0[k:k + 1], np.eye(5)[None], init[k], None, *args,
+                                               sigma=(tab.xi, tab.wm, tab.wc), want_ell=True, threads=1)
+        assert np.array_equal(bm[off[k]:off[k + 1]], m) and np.array_equal(bL[off[k]:off[k + 1]], L)
+        assert be[k] == e[0]
+
+
 # ---------------------------------------------------------------- pathwise sampler
 def _linear_smoothing_setup(seed, T=25):
     np.random.seed(seed)
