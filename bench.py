@@ -249,17 +249,10 @@ def linear_cv_case(steps, warmup):
             "mean_us": mean * 1e6, "median_us": med * 1e6}
 
 
-# Reference-published FP32 stability: % of 100 runs whose FP32 log-likelihood
-# is NaN after 20 iterations (notebooks/robustness_data/R1_rts_fl32_extended_ell.csv
-# and R1_rts_fl32_cubature_ell.csv, protocol notebooks/robustness_100runs.py).
+# Prefix lengths of the reference's FP32 robustness protocol
+# (notebooks/robustness_100runs.py:48).
 REF_FP32_NAN_T = [20, 30, 40, 50, 80, 100, 200, 300, 400, 500, 1000, 1500, 2000, 2500, 3000, 3500, 4000,
                   4500, 5000, 5500, 6000, 6500, 7000, 7500]
-REF_FP32_NAN_PCT = {
-    "extended_sqrt": [0] * 24,
-    "extended_std": [0] * 15 + [1, 0, 1, 1, 1, 1, 1, 2, 2],
-    "cubature_sqrt": [0] * 17 + [1, 0, 0, 0, 1, 2, 2],
-    "cubature_std": [0, 0, 2, 4, 19] + [100] * 19,
-}
 
 
 # Reference-published GPU times at T=4096 (notebooks/new_outputs/rtx-GPU-all-methods.csv
@@ -459,56 +452,70 @@ def fp32_stability(Ts, n_runs=100, n_iter=20):
     """The reference's FP32 robustness protocol (notebooks/robustness_100runs.py:
     100 bearings-only runs, each a prefix of length T of one T=8000 simulation;
     m0 = [-4, -1, 2, 7, 3], P0 = I; nominal [-1, -1, 6, 4, 2] with I at every
-    state; 20 iterations; log-likelihood at the final nominal) in FP32, square-root
-    and covariance form, extended and cubature.  The reference's runs are
-    unseeded; here run j is simulated with seed j.  All runs of one T go into
-    one ragged batch; the log-likelihood accumulates in FP64.  Reports the % of
-    runs with a non-finite log-likelihood next to the reference's published %,
-    and the % of runs whose FP64 iteration (same kernels) diverges -- its final
-    smoothed mean exceeds 1e3 in magnitude or is not finite -- the runs on which
-    FP32 can only fail."""
+    state; criterion i < 20; log-likelihood at the final nominal) in FP32,
+    square-root and covariance form, extended and cubature, on the reference's
+    own 100 runs (tests/golden/robustness_runs.npz, truncated to the
+    protocol's longest prefix).  All runs of one T go into one ragged batch;
+    the log-likelihood accumulates in FP64.  Reports per T: the % of runs with a
+    non-finite FP32 log-likelihood next to the reference's (its FP32 tables),
+    the % of runs whose FP64 iteration (same kernels) diverges -- final smoothed
+    mean beyond 1e3 in magnitude or not finite -- and, on the runs finite in
+    both, the median relative difference of our FP32 log-likelihood from the
+    reference's FP32 one and from our FP64 one."""
     import torch
     from parsmooth import FunctionalModel, MVNSqrt, MVNStandard
     from parsmooth.batch import IteratedSmoother
-    from parsmooth.data import bearings_data
     from parsmooth.linearization import cubature, extended
     from parsmooth.models import make_parameters
     dev = torch.device("cuda", 0)
-    Tmax = max(Ts)
-    sims = [bearings_data(np.array([0.1, 0.2, 1.0, 0.0]), 0.01, 0.5, max(Tmax, 8000), (-1.5, 0.5), (1.0, 1.0),
-                          random_state=j + 1)[2] for j in range(n_runs)]
+    fx = np.load(os.path.join(ROOT, "tests", "golden", "robustness_runs.npz"), allow_pickle=False)
+    runs = fx["ys"][:n_runs]
+    fixture_T = list(fx["Ts"])
     Q, R, obs_f, trans_f = make_parameters(0.01, 0.1, 0.5, 0.01, (-1.5, 0.5), (1.0, 1.0))
-    t = lambda a: torch.as_tensor(np.asarray(a, dtype=np.float64)).to(dev, torch.float32)
     m0 = np.array([-4.0, -1.0, 2.0, 7.0, 3.0])
-    res = {"protocol": "100 runs x 20 iterations, FP32, % runs with non-finite log-likelihood",
-           "T": list(Ts), "ours": {}, "reference": {}, "fp64_diverged": {}}
-    t64 = lambda a: torch.as_tensor(np.asarray(a, dtype=np.float64)).to(dev, torch.float64)
+    res = {"protocol": "the reference's 100 runs x 20 iterations, FP32, % runs with non-finite log-likelihood",
+           "T": list(Ts), "ours": {}, "reference": {}, "fp64_diverged": {}, "ell_vs_reference_fp32": {},
+           "ell_vs_ours_fp64": {}}
     for method_name, method in (("extended", extended), ("cubature", cubature)):
         for form, sqrt in (("sqrt", True), ("std", False)):
             cls = MVNSqrt if sqrt else MVNStandard
             key = f"{method_name}_{form}"
-            ours, div64 = [], []
+            ours, div64, vref, v64 = [], [], [], []
             for T in Ts:
-                for dt_, tt in ((torch.float32, t), (torch.float64, t64)):
+                ells = {}
+                for dt_ in (torch.float32, torch.float64):
+                    tt = lambda a: torch.as_tensor(np.asarray(a, dtype=np.float64)).to(dev, dt_)   # noqa: E731
                     trans = FunctionalModel(trans_f, cls(tt(np.zeros(5)), tt(np.linalg.cholesky(Q) if sqrt else Q)))
                     obsm = FunctionalModel(obs_f, cls(tt(np.zeros(2)), tt(np.linalg.cholesky(R) if sqrt else R)))
                     r = IteratedSmoother([T] * n_runs, trans, obsm, method, dtype=dt_, sqrt=sqrt)
-                    r.set_data(tt(np.concatenate([y[:T] for y in sims])), tt(np.tile(m0, (n_runs, 1))),
+                    r.set_data(tt(runs[:, :T].reshape(-1, 2)), tt(np.tile(m0, (n_runs, 1))),
                                tt(np.tile(np.eye(5), (n_runs, 1, 1))))
                     M = r.prob.n_states
                     init = (tt(np.tile(np.array([-1.0, -1.0, 6.0, 4.0, 2.0]), (M, 1))),
                             tt(np.tile(np.eye(5), (M, 1, 1))))
                     mean, _, ell = r.run(n_iter, init, use_graph=False, return_loglikelihood=True)
-                    if dt_ == torch.float32:
-                        ours.append(float(100.0 * np.mean(~np.isfinite(ell.double().cpu().numpy()))))
-                    else:
+                    ells[dt_] = ell.double().cpu().numpy()
+                    if dt_ == torch.float64:
                         big = [not bool(torch.isfinite(x).all()) or float(x.abs().max()) > 1e3 for x in r.split(mean)]
                         div64.append(float(100.0 * np.mean(big)))
                     del r, init
+                e32, e64 = ells[torch.float32], ells[torch.float64]
+                ours.append(float(100.0 * np.mean(~np.isfinite(e32))))
+                ref_row = fx[f"ell_{key}"][fixture_T.index(T)].astype(np.float64) if T in fixture_T else None
+                if ref_row is not None:
+                    ok = np.isfinite(e32) & np.isfinite(ref_row)
+                    vref.append(float(np.median(np.abs(e32[ok] - ref_row[ok]) / np.abs(ref_row[ok]))) if ok.any()
+                                else None)
+                else:
+                    vref.append(None)
+                ok = np.isfinite(e32) & np.isfinite(e64)
+                v64.append(float(np.median(np.abs(e32[ok] - e64[ok]) / np.abs(e64[ok]))) if ok.any() else None)
             res["ours"][key] = ours
             res["fp64_diverged"][key] = div64
-            res["reference"][key] = [REF_FP32_NAN_PCT[key][REF_FP32_NAN_T.index(T)] if T in REF_FP32_NAN_T
-                                     else None for T in Ts]
+            res["ell_vs_reference_fp32"][key] = vref
+            res["ell_vs_ours_fp64"][key] = v64
+            res["reference"][key] = [float(100.0 * np.mean(np.isnan(fx[f"ell_{key}"][fixture_T.index(T)])))
+                                     if T in fixture_T else None for T in Ts]
     torch.cuda.empty_cache()
     return res
 

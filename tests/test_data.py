@@ -72,3 +72,24 @@ def test_population_parameters_values():
     assert float(co(x)) == pytest.approx(10 * np.exp(0.7), rel=1e-14)
     assert float(cho(x)) == pytest.approx(np.sqrt(10 * np.exp(0.7)), rel=1e-14)
     assert tuple(co(x).shape) == (1, 1) and tuple(cho(x).shape) == (1, 1)
+
+
+def test_robustness_fixture_reproduces_the_published_nan_rates():
+    """tests/golden/robustness_runs.npz holds the reference's 100 FP32
+    robustness runs and its log-likelihood tables; the % of NaN runs per T of
+    those tables are the published rates (R1_rts_fl32_*_ell.csv of the
+    reference's notebooks/robustness_data, restated here)."""
+    import os
+    d = np.load(os.path.join(os.path.dirname(__file__), "golden", "robustness_runs.npz"), allow_pickle=False)
+    published = {
+        "extended_sqrt": [0] * 24,
+        "extended_std": [0] * 15 + [1, 0, 1, 1, 1, 1, 1, 2, 2],
+        "cubature_sqrt": [0] * 17 + [1, 0, 0, 0, 1, 2, 2],
+        "cubature_std": [0, 0, 2, 4, 19] + [100] * 19,
+    }
+    assert d["ys"].shape == (100, 7500, 2) and d["ys"].dtype == np.float32
+    assert list(d["Ts"]) == [20, 30, 40, 50, 80, 100, 200, 300, 400, 500, 1000, 1500, 2000, 2500, 3000, 3500,
+                             4000, 4500, 5000, 5500, 6000, 6500, 7000, 7500]
+    for key, pct in published.items():
+        got = 100.0 * np.mean(np.isnan(d[f"ell_{key}"]), axis=1)
+        assert np.array_equal(got, np.asarray(pct, np.float64)), (key, got)
