@@ -262,6 +262,69 @@ int psm_iterated_smoother_graph(const psm_problem_t* prob, const psm_model_t* tr
                                 double* ell, void* workspace, size_t workspace_bytes,
                                 int32_t use_graph, void* stream);
 
+/* ---------------------------------------------------------------------------
+ * Reverse-mode adjoint (vector-Jacobian products) of one filter + smoother
+ * pass at a nominal trajectory and of its log-likelihood, for the implicit
+ * gradient of iterated smoothing.  Replaces the custom VJP of the fixed point,
+ * parsmooth/_utils.py:118-133 (_fixed_point_rev / _rev_iter: vjp of
+ * filter_smoother with respect to the nominal and the model parameters) and
+ * jax.grad of the log-likelihood of parsmooth/parallel/_filtering.py:53-60.
+ * Square-root form, PSM_F32 / PSM_F64 primal (all adjoint arithmetic and
+ * every cotangent array is FP64), models the kernels evaluate.  Cotangents of
+ * chol factors are lower triangular (the factors' lower triangles are the
+ * variables).
+ *
+ * Parameter cotangents theta_bar: (n_seq, PSM_ADJ_NTH(nx, ny)) doubles per
+ * call, per sequence, laid out as [transition params (nx*nx slot; the model
+ * kind's parameter count is a prefix: linear nx*nx, coordinated turn 1) |
+ * transition noise mean (nx) | transition noise chol (nx*nx) | observation
+ * params (ny*nx slot: linear ny*nx, bearings 4) | observation noise mean (ny)
+ * | observation noise chol (ny*ny) | prior mean m0 (nx) | prior chol L0
+ * (nx*nx)].  theta_bar may be NULL.
+ *
+ * Workspace: caller-owned, psm_adjoint_workspace_bytes; psm_adjoint_prepare
+ * writes the checkpoint there (the primal pass's filtered and smoothed
+ * covariances, gains and filter-element matrices at the nominal, FP64), which
+ * the VJP calls read, so it must stay intact until the last VJP of that
+ * nominal.
+ * ------------------------------------------------------------------------- */
+#define PSM_ADJ_NTH(nx, ny) (3 * (nx) * (nx) + 2 * (nx) + (ny) * (nx) + (ny) + (ny) * (ny))
+
+size_t psm_adjoint_workspace_bytes(const psm_problem_t* prob);
+
+/* The primal pass at (nom_m, nom_L) -- filter then smoother, as
+ * psm_filter_smoother, writing filt_* and out_* -- followed by the adjoint
+ * checkpoint of that pass in aws.  ws / wsb: the primal's workspace. */
+int psm_adjoint_prepare(const psm_problem_t* prob, const psm_model_t* trans, const psm_model_t* obs,
+                        const psm_sigma_t* sigma, const void* ys, const void* m0, const void* L0,
+                        const void* nom_m, const void* nom_L, void* filt_m, void* filt_L, void* out_m,
+                        void* out_L, void* workspace, size_t workspace_bytes, void* aws, size_t aws_bytes,
+                        void* stream);
+
+/* J^T u of the pass prepared at (nom_m, nom_L): for cotangents (cot_m, cot_L)
+ * of its smoothed trajectory (sm_m, sm_L, as psm_adjoint_prepare returned
+ * it), the cotangents of the nominal trajectory (out_m (n_states, nx), out_L
+ * (n_states, nx, nx)), plus (add_m, add_L) when those are given (both or
+ * neither), and of the model parameters (theta_bar).  One call evaluates both
+ * products whatever the number of parameters: the reverse sum
+ * u <- x_bar + J_x^T u of the fixed point is add = x_bar. */
+int psm_filter_smoother_vjp(const psm_problem_t* prob, const psm_model_t* trans, const psm_model_t* obs,
+                            const psm_sigma_t* sigma, const void* ys, const void* m0, const void* L0,
+                            const void* nom_m, const void* nom_L, const void* sm_m, const void* sm_L,
+                            const double* cot_m, const double* cot_L, const double* add_m,
+                            const double* add_L, double* out_m, double* out_L, double* theta_bar,
+                            void* aws, size_t aws_bytes, void* stream);
+
+/* Gradient of the per-sequence log-likelihood of the filter at (nom_m,
+ * nom_L) (the ell of psm_filter) with respect to the nominal trajectory
+ * (out_m, out_L) and the model parameters (theta_bar), by Fisher's identity
+ * over the smoothed distribution of the linearized model (sm_m, sm_L from
+ * psm_adjoint_prepare). */
+int psm_filter_vjp(const psm_problem_t* prob, const psm_model_t* trans, const psm_model_t* obs,
+                   const psm_sigma_t* sigma, const void* ys, const void* m0, const void* L0,
+                   const void* nom_m, const void* nom_L, const void* sm_m, const void* sm_L, double* out_m,
+                   double* out_L, double* theta_bar, void* aws, size_t aws_bytes, void* stream);
+
 /* Pathwise samples of the smoothing distribution at a nominal trajectory.
  * Replaces parsmooth/_pathwise_sampler.py:13-38 (parallel sampling, reached
  * from parsmooth/methods.py:79-91) with its elements :64-127.  From the

@@ -125,11 +125,11 @@ __global__ __launch_bounds__(1024) void k_mean_sq_diff(int64_t n, const R* __res
 // be captured) share one non-blocking side stream, ordered after and before
 // the caller's stream by two events created once.  At most kMaxGraphs graphs
 // are kept; the least recently used one is dropped (after its last launch has
-// finished) to make room.
+// finished, which the entry's own event tells) to make room.
 struct GraphEntry {
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
-  hipStream_t last = nullptr;  // stream of the last launch
+  hipEvent_t done = nullptr;   // recorded after every launch
   uint64_t used = 0;           // graph-clock value of the last launch
 };
 constexpr size_t kMaxGraphs = 64;
@@ -172,7 +172,10 @@ static bool default_stream(hipStream_t st) {
 }
 
 static void drop_graph(GraphEntry& ge) {
-  if (ge.last) (void)hipStreamSynchronize(ge.last);
+  if (ge.done) {
+    (void)hipEventSynchronize(ge.done);
+    (void)hipEventDestroy(ge.done);
+  }
   if (ge.exec) (void)hipGraphExecDestroy(ge.exec);
   if (ge.graph) (void)hipGraphDestroy(ge.graph);
 }
@@ -222,6 +225,13 @@ static int run_as_graph(std::vector<uint64_t>& key, hipStream_t st, Fn&& run) {
       set_error(std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
       return PSM_E_HIP;
     }
+    e = hipEventCreateWithFlags(&ge.done, hipEventDisableTiming);
+    if (e != hipSuccess) {
+      ge.done = nullptr;
+      drop_graph(ge);
+      set_error(std::string("hipEventCreate: ") + hipGetErrorString(e));
+      return PSM_E_HIP;
+    }
     it = g_graphs.emplace(key, ge).first;
   }
   hipError_t e;
@@ -234,7 +244,7 @@ static int run_as_graph(std::vector<uint64_t>& key, hipStream_t st, Fn&& run) {
   } else {
     e = hipGraphLaunch(it->second.exec, cs);
   }
-  it->second.last = cs;
+  (void)hipEventRecord(it->second.done, cs);
   it->second.used = ++g_graph_clock;
   if (e != hipSuccess) {
     set_error(std::string("hipGraphLaunch: ") + hipGetErrorString(e));
@@ -463,6 +473,78 @@ int psm_iterated_smoother_graph(const psm_problem_t* p, const psm_model_t* trans
     push(key, q);
   key.push_back((uint64_t)wsb);
   return run_as_graph(key, st, run);
+}
+
+// ---------------------------------------------------------------- reverse-mode adjoint
+size_t psm_adjoint_workspace_bytes(const psm_problem_t* p) {
+  if (check_problem(p)) return 0;
+  const OpsTable* t = need(p->dtype, p->form, p->nx, p->ny);
+  if (!t) return 0;
+  return t->adjoint_workspace_bytes(p);
+}
+
+int psm_adjoint_prepare(const psm_problem_t* p, const psm_model_t* trans, const psm_model_t* obs,
+                        const psm_sigma_t* sigma, const void* ys, const void* m0, const void* L0,
+                        const void* nom_m, const void* nom_L, void* filt_m, void* filt_L, void* out_m,
+                        void* out_L, void* ws, size_t wsb, void* aws, size_t awsb, void* stream) {
+  int rc = check_problem(p);
+  if (rc) return rc;
+  if ((rc = check_model(trans, p->nx, p->nx))) return rc;
+  if ((rc = check_model(obs, p->nx, p->ny))) return rc;
+  if (!ys || !m0 || !L0 || !nom_m || !nom_L || !filt_m || !filt_L || !out_m || !out_L || !aws) {
+    set_error("psm_adjoint_prepare: null array");
+    return PSM_E_ARG;
+  }
+  const OpsTable* t = need(p->dtype, p->form, p->nx, p->ny);
+  if (!t) return PSM_E_UNSUPPORTED;
+  hipStream_t st = (hipStream_t)stream;
+  rc = t->filter(p, trans, obs, sigma, ys, m0, L0, nom_m, nom_L, filt_m, filt_L, nullptr, ws, wsb, st, 0);
+  if (rc) return rc;
+  rc = t->smoother(p, trans, sigma, filt_m, filt_L, nom_m, nom_L, out_m, out_L, ws, wsb, st, 1);
+  if (rc) return rc;
+  return t->adjoint_prepare(p, trans, obs, sigma, nom_m, nom_L, filt_m, filt_L, out_m, out_L, aws, awsb, st);
+}
+
+static int adjoint_common(const psm_problem_t* p, const psm_model_t* trans, const psm_model_t* obs,
+                          const OpsTable** t) {
+  int rc = check_problem(p);
+  if (rc) return rc;
+  if ((rc = check_model(trans, p->nx, p->nx))) return rc;
+  if ((rc = check_model(obs, p->nx, p->ny))) return rc;
+  *t = need(p->dtype, p->form, p->nx, p->ny);
+  return *t ? PSM_OK : PSM_E_UNSUPPORTED;
+}
+
+int psm_filter_smoother_vjp(const psm_problem_t* p, const psm_model_t* trans, const psm_model_t* obs,
+                            const psm_sigma_t* sigma, const void* ys, const void* m0, const void* L0,
+                            const void* nom_m, const void* nom_L, const void* sm_m, const void* sm_L,
+                            const double* cot_m, const double* cot_L, const double* add_m,
+                            const double* add_L, double* out_m, double* out_L, double* theta_bar,
+                            void* aws, size_t awsb, void* stream) {
+  const OpsTable* t = nullptr;
+  int rc = adjoint_common(p, trans, obs, &t);
+  if (rc) return rc;
+  if (!cot_m || !cot_L || !out_m || !out_L || !sm_m || !sm_L || !aws || (!add_m) != (!add_L)) {
+    set_error("psm_filter_smoother_vjp: null array");
+    return PSM_E_ARG;
+  }
+  return t->adjoint_vjp(p, 0, trans, obs, sigma, ys, m0, L0, nom_m, nom_L, sm_m, sm_L, cot_m, cot_L, add_m,
+                        add_L, out_m, out_L, theta_bar, aws, awsb, (hipStream_t)stream);
+}
+
+int psm_filter_vjp(const psm_problem_t* p, const psm_model_t* trans, const psm_model_t* obs,
+                   const psm_sigma_t* sigma, const void* ys, const void* m0, const void* L0,
+                   const void* nom_m, const void* nom_L, const void* sm_m, const void* sm_L, double* out_m,
+                   double* out_L, double* theta_bar, void* aws, size_t awsb, void* stream) {
+  const OpsTable* t = nullptr;
+  int rc = adjoint_common(p, trans, obs, &t);
+  if (rc) return rc;
+  if (!out_m || !out_L || !sm_m || !sm_L || !aws) {
+    set_error("psm_filter_vjp: null array");
+    return PSM_E_ARG;
+  }
+  return t->adjoint_vjp(p, 1, trans, obs, sigma, ys, m0, L0, nom_m, nom_L, sm_m, sm_L, nullptr, nullptr,
+                        nullptr, nullptr, out_m, out_L, theta_bar, aws, awsb, (hipStream_t)stream);
 }
 
 size_t psm_sample_workspace_bytes(const psm_problem_t* p, int32_t n_samples) {

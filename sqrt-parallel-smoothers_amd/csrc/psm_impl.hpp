@@ -7,6 +7,7 @@
 #include <cstdlib>
 #include <string>
 
+#include "psm_adjoint.hpp"
 #include "psm_kernels.hpp"
 #include "psm_sample.hpp"
 #include "psm_seq.hpp"
@@ -202,6 +203,15 @@ struct OpsTable {
   // level-0 workgroup size << 33; a captured launch sequence is valid only for
   // the same signature
   uint64_t (*plan_signature)(const psm_problem_t*);
+  // reverse-mode adjoint (square-root form, float / double primal)
+  size_t (*adjoint_workspace_bytes)(const psm_problem_t*);
+  int (*adjoint_prepare)(const psm_problem_t*, const psm_model_t*, const psm_model_t*, const psm_sigma_t*,
+                         const void*, const void*, const void*, const void*, const void*, const void*,
+                         void*, size_t, hipStream_t);
+  int (*adjoint_vjp)(const psm_problem_t*, int, const psm_model_t*, const psm_model_t*, const psm_sigma_t*,
+                     const void*, const void*, const void*, const void*, const void*, const void*, const void*,
+                     const double*, const double*, const double*, const double*, double*, double*, double*,
+                     void*, size_t, hipStream_t);
 };
 
 template <typename R, int NX, int NY, bool SQRT>
@@ -406,7 +416,7 @@ struct Impl {
 
   static int linearize_states(const Plan& pl, const psm_model_t* tr, const psm_model_t* ob,
                               const psm_sigma_t* sg, const void* nm, const void* nL, char* ws,
-                              hipStream_t st) {
+                              hipStream_t st, const Fallback& fb) {
     int use = 0;
     SigmaView<R> sv = sigma_view<R>(sg, &use);
     ModelView<R> trv = model_view<R>(tr);
@@ -417,7 +427,7 @@ struct Impl {
     hipLaunchKernelGGL((k_linearize_states<R, NX, NY, SQRT>), dim3(grid_for(pl.Mp, kLaneWG)),
                        dim3(kLaneWG), 0, st, pl.M, pl.Mp, pl.chunk, trv, obv,
                        ob ? 1 : 0, sv, use,
-                       (const R*)nm, (const R*)nL, (R*)(ws + pl.off_lin));
+                       (const R*)nm, (const R*)nL, (R*)(ws + pl.off_lin), fb);
     return hip_check("k_linearize_states");
   }
 
@@ -435,7 +445,7 @@ struct Impl {
     int rc;
     const Fallback fb{p->fallback_ratio, p->fallback_count};
     if (!lin_done) {
-      rc = linearize_states(pl, tr, ob, sg, nm, nL, ws, st);
+      rc = linearize_states(pl, tr, ob, sg, nm, nL, ws, st, fb);
       if (rc) return rc;
     }
     const R* lin = (const R*)(ws + pl.off_lin);
@@ -482,7 +492,7 @@ struct Impl {
     int rc;
     const Fallback fb{p->fallback_ratio, p->fallback_count};
     if (!lin_done) {
-      rc = linearize_states(pl, tr, nullptr, sg, nm, nL, ws, st);
+      rc = linearize_states(pl, tr, nullptr, sg, nm, nL, ws, st, fb);
       if (rc) return rc;
     }
     {
@@ -541,9 +551,9 @@ struct Impl {
     }
     if (pl.M <= 0 || S <= 0) return PSM_OK;
     char* ws = (char*)wsv;
-    int rc = linearize_states(pl, tr, nullptr, sg, nm, nL, ws, st);
-    if (rc) return rc;
     const Fallback fb{p->fallback_ratio, p->fallback_count};
+    int rc = linearize_states(pl, tr, nullptr, sg, nm, nL, ws, st, fb);
+    if (rc) return rc;
     // smoother elements at every state (the chunk aggregates it also forms are not used)
     hipLaunchKernelGGL((k_smoother_reduce<R, NX, NY, SQRT>), dim3(grid_for(pl.nchunks, kLaneWG)),
                        dim3(kLaneWG), 0, st, pl.M, pl.Mp, pl.chunk, pl.nchunks, p->state_offsets,
@@ -612,12 +622,210 @@ struct Impl {
     }
   }
 
+  // ---------------------------------------------------------------- adjoint
+  static constexpr bool kAdj = SQRT && (std::is_same<R, float>::value || std::is_same<R, double>::value);
+  struct AdjPlan {
+    int64_t M = 0, Mp = 0;
+    size_t off_ck = 0, off_pa = 0, off_lc = 0, off_th = 0;
+    int nlv = 0;               // aggregate levels above the states
+    int64_t lvn[16] = {}, lvst[16] = {};
+    size_t off_lv[16] = {};
+    size_t total = 0;
+  };
+  static AdjPlan adj_plan(const psm_problem_t* p) {
+    AdjPlan a;
+    constexpr int C = adj::kChunk;
+    a.M = p->n_states;
+    a.Mp = state_stride(a.M, C);
+    size_t off = 0;
+    const size_t d = sizeof(double);
+    a.off_ck = off;
+    off = align_up(off + (size_t)adj::Ckp<NX>::N * a.Mp * d);
+    a.off_pa = off;
+    off = align_up(off + (size_t)adj::Pas<NX>::N * a.Mp * d);
+    a.off_lc = off;
+    off = align_up(off + (size_t)adj::Lco<NX, NY>::N * a.Mp * d);
+    a.off_th = off;
+    off = align_up(off + (size_t)adj::Th<NX, NY>::N * a.Mp * d);
+    // aggregate levels: level 1 holds ceil(M / C) items, ... up to one chunk
+    int64_t n = a.M;
+    while (n > C && a.nlv < 15) {
+      n = (n + C - 1) / C;
+      ++a.nlv;
+      a.lvn[a.nlv] = n;
+      a.lvst[a.nlv] = state_stride(n, C);
+      a.off_lv[a.nlv] = off;
+      off = align_up(off + (size_t)adj::Sc<NX, true>::NQ * a.lvst[a.nlv] * d);
+    }
+    a.lvn[0] = a.M;
+    a.lvst[0] = a.Mp;
+    a.total = off;
+    return a;
+  }
+  static size_t adjoint_workspace_bytes(const psm_problem_t* p) {
+    if constexpr (!kAdj) return 0;
+    else return adj_plan(p).total;
+  }
+  static ModelView<double, R> model_view_d(const psm_model_t* m) {
+    return ModelView<double, R>{m->kind, static_cast<const R*>(m->params), static_cast<const R*>(m->q_mean),
+                                static_cast<const R*>(m->q_chol)};
+  }
+  static int adj_check(const psm_problem_t* p, const psm_model_t* tr, const psm_model_t* ob, size_t awsb) {
+    if constexpr (!kAdj) {
+      set_error("the adjoint supports the square-root form in PSM_F32 / PSM_F64 only");
+      return PSM_E_UNSUPPORTED;
+    } else {
+      for (const psm_model_t* m : {tr, ob}) {
+        if (m->kind == PSM_MODEL_TABULATED || m->kind == PSM_MODEL_CONDITIONAL) {
+          set_error("the adjoint needs models the kernels evaluate (not tabulated or conditional-moments models)");
+          return PSM_E_UNSUPPORTED;
+        }
+      }
+      if (awsb < adj_plan(p).total) {
+        set_error("adjoint workspace too small");
+        return PSM_E_WORKSPACE;
+      }
+      return PSM_OK;
+    }
+  }
+
+  // Checkpoint of the pass at nominal (nm, nL) whose filtered (fm, fL) and
+  // smoothed (sm, sL) trajectories the caller computed.
+  static int adjoint_prepare(const psm_problem_t* p, const psm_model_t* tr, const psm_model_t* ob,
+                             const psm_sigma_t* sg, const void* nm, const void* nL, const void* fm,
+                             const void* fL, const void* sm, const void* sL, void* awsv, size_t awsb,
+                             hipStream_t st) {
+    int rc = adj_check(p, tr, ob, awsb);
+    if (rc) return rc;
+    if constexpr (kAdj) {
+      (void)sm;
+      const AdjPlan a = adj_plan(p);
+      if (a.M <= 0) return PSM_OK;
+      int use = 0;
+      SigmaView<double> sv = sigma_view<double>(sg, &use);
+      ModelView<double, R> trv = model_view_d(tr), obv = model_view_d(ob);
+      char* aws = (char*)awsv;
+      TimedScope ts_("k_adj_prep", st);
+      hipLaunchKernelGGL((adj::k_adj_prep<R, NX, NY>), dim3(grid_for(a.Mp, 64)), dim3(64), 0, st, a.M, a.Mp,
+                         p->state_offsets, p->n_seq, trv, obv, sv, use, (const R*)nm, (const R*)nL,
+                         (const R*)fm, (const R*)fL, (const R*)sL, (double*)(aws + a.off_ck));
+      return hip_check("k_adj_prep");
+    }
+    return PSM_OK;
+  }
+
+  // chunked scan of one recursion (levels above the states as planned)
+  template <bool CONG, int SRC>
+  static int adj_scan(const AdjPlan& a, char* aws, hipStream_t st) {
+    using namespace adj;
+    const double* ck = (const double*)(aws + a.off_ck);
+    double* pa = (double*)(aws + a.off_pa);
+    auto lvbuf = [&](int k) { return (double*)(aws + a.off_lv[k]); };
+    const ScanIO<NX, CONG, SRC> io0{ck, pa, nullptr, a.Mp, 0, a.M};
+    constexpr int C = kChunk;
+    // up-sweep: aggregates of level k into level k + 1
+    for (int k = 0; k < a.nlv; ++k) {
+      const unsigned g = grid_for((a.lvn[k] + C - 1) / C, 64);
+      if (k == 0) {
+        hipLaunchKernelGGL((k_ascan_reduce<NX, CONG, SRC>), dim3(g), dim3(64), 0, st, a.lvn[0], io0,
+                           lvbuf(1), a.lvst[1]);
+      } else {
+        const ScanIO<NX, CONG, SRC_LEVEL> io{ck, pa, lvbuf(k), a.Mp, a.lvst[k], a.M};
+        hipLaunchKernelGGL((k_ascan_reduce<NX, CONG, SRC_LEVEL>), dim3(g), dim3(64), 0, st, a.lvn[k], io,
+                           lvbuf(k + 1), a.lvst[k + 1]);
+      }
+      int rc = hip_check("k_ascan_reduce");
+      if (rc) return rc;
+    }
+    // down-sweep: the top level from zero, then every level from the one above
+    for (int k = a.nlv; k >= 0; --k) {
+      const unsigned g = grid_for((a.lvn[k] + C - 1) / C, 64);
+      const double* up = k == a.nlv ? nullptr : lvbuf(k + 1);
+      const int64_t ust = k == a.nlv ? 0 : a.lvst[k + 1];
+      if (k == 0) {
+        hipLaunchKernelGGL((k_ascan_apply<NX, CONG, SRC>), dim3(g), dim3(64), 0, st, a.lvn[0], io0, up, ust);
+      } else {
+        const ScanIO<NX, CONG, SRC_LEVEL> io{ck, pa, lvbuf(k), a.Mp, a.lvst[k], a.M};
+        hipLaunchKernelGGL((k_ascan_apply<NX, CONG, SRC_LEVEL>), dim3(g), dim3(64), 0, st, a.lvn[k], io, up,
+                           ust);
+      }
+      int rc = hip_check("k_ascan_apply");
+      if (rc) return rc;
+    }
+    return PSM_OK;
+  }
+
+  // mode 0: cotangents (cot_m, cot_L) of the smoothed trajectory -> nominal
+  // cotangents (out_m, out_L) [+ (add_m, add_L)] and parameter cotangents theta
+  // (n_seq x NTH); mode 1: the same for the pass's log-likelihood (cot_* unused).
+  static int adjoint_vjp(const psm_problem_t* p, int mode, const psm_model_t* tr, const psm_model_t* ob,
+                         const psm_sigma_t* sg, const void* ys, const void* m0, const void* L0, const void* nm,
+                         const void* nL, const void* sm, const void* sL, const double* cot_m,
+                         const double* cot_L, const double* add_m, const double* add_L, double* out_m,
+                         double* out_L, double* theta, void* awsv, size_t awsb, hipStream_t st) {
+    int rc = adj_check(p, tr, ob, awsb);
+    if (rc) return rc;
+    if constexpr (kAdj) {
+      using namespace adj;
+      const AdjPlan a = adj_plan(p);
+      if (a.M <= 0) return PSM_OK;
+      int use = 0;
+      SigmaView<double> sv = sigma_view<double>(sg, &use);
+      ModelView<double, R> trv = model_view_d(tr), obv = model_view_d(ob);
+      char* aws = (char*)awsv;
+      const double* ck = (const double*)(aws + a.off_ck);
+      double* pa = (double*)(aws + a.off_pa);
+      double* lc = (double*)(aws + a.off_lc);
+      double* th = (double*)(aws + a.off_th);
+      const unsigned g = grid_for(a.Mp, 64);
+      if (mode == MODE_OUTPUT) {
+        TimedScope ts_("k_adj_scans", st);
+        hipLaunchKernelGGL((k_adj_items<R, NX>), dim3(g), dim3(64), 0, st, a.M, a.Mp, (const R*)sL, cot_m,
+                           cot_L, ck, pa);
+        if ((rc = hip_check("k_adj_items"))) return rc;
+        if ((rc = adj_scan<false, SRC_FWD_VEC>(a, aws, st))) return rc;
+        hipLaunchKernelGGL((k_adj_items2<NX>), dim3(g), dim3(64), 0, st, a.M, a.Mp, ck, pa);
+        if ((rc = hip_check("k_adj_items2"))) return rc;
+        if ((rc = adj_scan<false, SRC_BWD_VEC>(a, aws, st))) return rc;
+        if ((rc = adj_scan<true, SRC_BWD_CONG>(a, aws, st))) return rc;
+        if ((rc = adj_scan<true, SRC_FWD_CONG>(a, aws, st))) return rc;
+      }
+      {
+        TimedScope ts_("k_adj_chain", st);
+        if (mode == MODE_OUTPUT)
+          hipLaunchKernelGGL((k_adj_chain<R, NX, NY, MODE_OUTPUT>), dim3(g), dim3(64), 0, st, a.M, a.Mp,
+                             p->state_offsets, p->n_seq, trv, obv, sv, use, (const R*)ys, (const R*)m0,
+                             (const R*)L0, (const R*)nm, (const R*)nL, (const R*)sm, ck, (const double*)pa, lc,
+                             th, a.Mp);
+        else
+          hipLaunchKernelGGL((k_adj_chain<R, NX, NY, MODE_FISHER>), dim3(g), dim3(64), 0, st, a.M, a.Mp,
+                             p->state_offsets, p->n_seq, trv, obv, sv, use, (const R*)ys, (const R*)m0,
+                             (const R*)L0, (const R*)nm, (const R*)nL, (const R*)sm, ck, (const double*)pa, lc,
+                             th, a.Mp);
+        if ((rc = hip_check("k_adj_chain"))) return rc;
+      }
+      {
+        TimedScope ts_("k_adj_lin", st);
+        hipLaunchKernelGGL((k_adj_lin<R, NX, NY>), dim3(g), dim3(64), 0, st, a.M, a.Mp, p->state_offsets,
+                           p->n_seq, trv, obv, sv, use, (const R*)nm, (const R*)nL, (const double*)lc, add_m,
+                           add_L, out_m, out_L, th, a.Mp);
+        if ((rc = hip_check("k_adj_lin"))) return rc;
+      }
+      if (theta != nullptr) {
+        hipLaunchKernelGGL((k_adj_theta<NX, NY>), dim3((unsigned)p->n_seq, (unsigned)Th<NX, NY>::N), dim3(256),
+                           0, st, a.Mp, p->state_offsets, (const double*)th, a.Mp, theta);
+        if ((rc = hip_check("k_adj_theta"))) return rc;
+      }
+    }
+    return PSM_OK;
+  }
+
   static const OpsTable* table() {
     static const OpsTable t = {&linearize_trans, &linearize_obs, &filter_elements,
                                &filter_combine,  &smoother_elements, &smoother_combine,
                                &workspace_bytes, &filter,          &smoother,
                                &sample_workspace_bytes, &sample, &seq_filter, &seq_smoother,
-                               &plan_signature};
+                               &plan_signature, &adjoint_workspace_bytes, &adjoint_prepare, &adjoint_vjp};
     return &t;
   }
 };

@@ -149,15 +149,18 @@ PSM_DEV void linearize_point(const ModelView<R, S>& md, const SigmaView<R>& sp, 
   }
 }
 
-// Linearization of the FP32 paths in FP64 arithmetic (FP32 in and out).
-// Sigma points: the statistical linear regression subtracts F P F^T from the
-// sigma-point covariance by rank-one downdates, and when the noise is small
-// against the sigma-point spread (chol Q ~ 6e-5 against a spread ~ 0.1 on the
-// bearings-only model) those downdates cancel in FP32 -- the factor loses all
-// its accuracy or the downdate fails outright.  Extended: model Jacobians
-// such as d/dw (1 - cos(w dt)) / w of the coordinated turn cancel in FP32 at
-// small turn rates (cos(w dt) rounds to 1 and the derivative doubles), and
-// b = f(m) - F m cancels at large |m|.
+// Linearization of the single-precision paths.  In FP32 arithmetic the
+// textbook formulas cancel: the sigma-point regression subtracts F P F^T from
+// the sigma-point covariance, which is ~0.1 against a process noise Q ~ 3e-9
+// on the bearings-only model; the coordinated turn's turn-rate derivatives are
+// differences of two numbers ~dt that agree to O((w dt)^2); b = f(m) - F m
+// cancels at large |m|.  The FP32 path therefore uses the cancellation-free
+// forms of psm_models.hpp (centred residuals, series in w dt, closed-form
+// offsets), all in FP32.  Rules with a negative weight and conditional-moments
+// models keep the reference's downdate form, evaluated in FP64 from the FP32
+// inputs and rounded back.  In square-root form, a regression noise factor
+// whose min|diag| / max|diag| falls below the fallback threshold is recomputed
+// that way too (and counted), like every other triangular factor.
 template <typename R> struct WiderOf { using type = R; };
 template <> struct WiderOf<float> { using type = double; };
 template <> struct WiderOf<DualT<float>> { using type = DualT<double>; };
@@ -174,20 +177,47 @@ PSM_DEV void widen(const B (&a)[M][N], A (&b)[M][N]) {
 }
 
 template <typename R, int NIN, int NOUT, bool SQRT>
+PSM_DEV void linearize_point_wide(const ModelView<R>& md, const SigmaView<R>& sp, int use_sigma,
+                                  const R (&m)[NIN], const R (&Lx)[NIN][NIN], R (&F)[NOUT][NIN],
+                                  R (&C)[NOUT][NOUT], R (&b)[NOUT]) {
+  using W = typename WiderOf<R>::type;
+  const ModelView<W, R> mw{md.kind, md.params, md.q_mean, md.q_chol, md.row, md.pt, md.npts};
+  const SigmaView<W> sw{sp.npts, sp.xi, sp.wm, sp.wc};
+  W mW[NIN], LW[NIN][NIN], FW[NOUT][NIN], CW[NOUT][NOUT], bW[NOUT];
+  widen(m, mW);
+  widen(Lx, LW);
+  linearize_point<W, NIN, NOUT, SQRT>(mw, sw, use_sigma, mW, LW, FW, CW, bW);
+  widen(FW, F);
+  widen(CW, C);
+  widen(bW, b);
+}
+
+template <typename R, int NIN, int NOUT, bool SQRT>
 PSM_DEV void linearize_point_w(const ModelView<R>& md, const SigmaView<R>& sp, int use_sigma,
                                const R (&m)[NIN], const R (&Lx)[NIN][NIN], R (&F)[NOUT][NIN],
-                               R (&C)[NOUT][NOUT], R (&b)[NOUT]) {
-  using W = typename WiderOf<R>::type;
-  if constexpr (!std::is_same<W, R>::value) {
-    const ModelView<W, R> mw{md.kind, md.params, md.q_mean, md.q_chol, md.row, md.pt, md.npts};
-    const SigmaView<W> sw{sp.npts, sp.xi, sp.wm, sp.wc};
-    W mW[NIN], LW[NIN][NIN], FW[NOUT][NIN], CW[NOUT][NOUT], bW[NOUT];
-    widen(m, mW);
-    widen(Lx, LW);
-    linearize_point<W, NIN, NOUT, SQRT>(mw, sw, use_sigma, mW, LW, FW, CW, bW);
-    widen(FW, F);
-    widen(CW, C);
-    widen(bW, b);
+                               R (&C)[NOUT][NOUT], R (&b)[NOUT], const Fallback& fb) {
+  if constexpr (IsSingle<R>::value) {
+    if (!use_sigma) {
+      linearize_extended<R, NIN, NOUT>(md, m, F, C, b);
+      return;
+    }
+    bool done = false;
+    if (md.kind != PSM_MODEL_CONDITIONAL) {
+      if constexpr (SQRT) {
+        done = linearize_sigma_centred<R, NIN, NOUT, true>(md, sp, m, Lx, F, C, b);
+        if constexpr (kFallback<R, SQRT>) {
+          if (done && fb.thr > 0.0f && ill_conditioned(C, fb.thr)) {
+            count_fallback(fb);
+            done = false;
+          }
+        }
+      } else {
+        R Lc[NIN][NIN];
+        cholesky(Lx, Lc);   // get_mvnsqrt (parsmooth/linearization/_common.py:9-14)
+        done = linearize_sigma_centred<R, NIN, NOUT, false>(md, sp, m, Lc, F, C, b);
+      }
+    }
+    if (!done) linearize_point_wide<R, NIN, NOUT, SQRT>(md, sp, use_sigma, m, Lx, F, C, b);
   } else {
     linearize_point<R, NIN, NOUT, SQRT>(md, sp, use_sigma, m, Lx, F, C, b);
   }
@@ -198,7 +228,8 @@ PSM_DEV void linearize_point_w(const ModelView<R>& md, const SigmaView<R>& sp, i
 template <typename R, int NX, int NY, bool SQRT>
 __global__ __launch_bounds__(kLaneWG) void k_linearize_states(
     int64_t M, int64_t Mp, int C, ModelView<R> tr, ModelView<R> ob, int do_obs, SigmaView<R> sp,
-    int use_sigma, const R* __restrict__ nom_m, const R* __restrict__ nom_L, R* __restrict__ lin) {
+    int use_sigma, const R* __restrict__ nom_m, const R* __restrict__ nom_L, R* __restrict__ lin,
+    Fallback fb) {
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= Mp) return;
   const int64_t g = (int64_t)C << 6;
@@ -213,7 +244,7 @@ __global__ __launch_bounds__(kLaneWG) void k_linearize_states(
   ld_aos(nom_L, i, L);
   {
     R F[NX][NX], Cq[NX][NX], b[NX];
-    linearize_point_w<R, NX, NX, SQRT>(tr, sp, use_sigma, m, L, F, Cq, b);
+    linearize_point_w<R, NX, NX, SQRT>(tr, sp, use_sigma, m, L, F, Cq, b, fb);
     int q = 0;
     st(lin, Mp, t, q, F);
     st(lin, Mp, t, q, Cq);
@@ -221,7 +252,7 @@ __global__ __launch_bounds__(kLaneWG) void k_linearize_states(
   }
   if (do_obs) {
     R H[NY][NX], Cr[NY][NY], c[NY];
-    linearize_point_w<R, NX, NY, SQRT>(ob, sp, use_sigma, m, L, H, Cr, c);
+    linearize_point_w<R, NX, NY, SQRT>(ob, sp, use_sigma, m, L, H, Cr, c, fb);
     int q = Step<R, NX, NY>::NTRANS;
     st(lin, Mp, t, q, H);
     st(lin, Mp, t, q, Cr);
@@ -644,7 +675,7 @@ __global__ __launch_bounds__(kLaneWG) void k_linearize_points(
   R m[NIN], L[NIN][NIN], Fo[NOUT][NIN], Co[NOUT][NOUT], bo[NOUT];
   ld_aos(xm, i, m);
   ld_aos(xL, i, L);
-  linearize_point_w<R, NIN, NOUT, SQRT>(md, sp, use_sigma, m, L, Fo, Co, bo);
+  linearize_point_w<R, NIN, NOUT, SQRT>(md, sp, use_sigma, m, L, Fo, Co, bo, Fallback{0.0f, nullptr});
   st_aos(F, i, Fo);
   st_aos(C, i, Co);
   st_aos(b, i, bo);

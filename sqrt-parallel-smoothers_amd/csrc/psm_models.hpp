@@ -56,6 +56,57 @@ template <typename B> PSM_DEV DualT<B> atan2_(DualT<B> y, DualT<B> x) {
   return {atan2_(y.v, x.v), (x.v * y.d - y.v * x.d) / q};
 }
 
+// Single-precision arithmetic types (FP32 values, FP32 tangents): their
+// linearizations use the cancellation-free formulas below (the coordinated
+// turn's terms are also used by every tangent type).
+template <typename R> struct IsSingle : std::false_type {};
+template <> struct IsSingle<float> : std::true_type {};
+template <> struct IsSingle<DualT<float>> : std::true_type {};
+
+// The turn-rate terms of the coordinated turn, x = w dt:
+//   a = sin(x)/w = dt sinc(x),        bb = (cos(x) - 1)/w = -dt g(x),
+//   da/dw = dt^2 sinc'(x),            db/dw = -dt^2 g'(x),
+// with sinc(x) = sin(x)/x and g(x) = (1 - cos x)/x.  The textbook quotients
+// cancel in FP32 at small x (dt cos(x) - sin(x)/w is a difference of two
+// numbers ~dt that agree to O(x^2)), so |x| < 0.5 uses the Taylor series of
+// sinc, sinc', g, g' (through x^9: truncation < 3e-11) and larger |x| the closed
+// forms with 1 - cos x = 2 sin^2(x/2).  |w| < 1e-6 keeps the reference's branch
+// (tests/bearings/bearings_utils.py:30-37): a = dt, bb = 0, zero w-derivatives.
+template <typename R>
+PSM_DEV void ct_terms_single(R w, R dt, R& a, R& bb, R& da, R& db, R& c, R& s) {
+  const R x = w * dt;
+  c = cos_(x);
+  s = sin_(x);
+  if (abs_(w) < R(1e-6)) {
+    a = dt;
+    bb = R(0);
+    da = R(0);
+    db = R(0);
+    return;
+  }
+  if (abs_(x) < R(0.5)) {
+    const R x2 = x * x;
+    const R sinc = R(1) + x2 * (R(-1.0 / 6) + x2 * (R(1.0 / 120) + x2 * (R(-1.0 / 5040) + x2 * R(1.0 / 362880))));
+    const R dsinc = x * (R(-1.0 / 3) + x2 * (R(1.0 / 30) + x2 * (R(-1.0 / 840) +
+                                                               x2 * (R(1.0 / 45360) + x2 * R(-1.0 / 3991680)))));
+    const R g = x * (R(0.5) + x2 * (R(-1.0 / 24) + x2 * (R(1.0 / 720) + x2 * (R(-1.0 / 40320) +
+                                                                             x2 * R(1.0 / 3628800)))));
+    const R dg = R(0.5) + x2 * (R(-1.0 / 8) + x2 * (R(1.0 / 144) + x2 * (R(-1.0 / 5760) + x2 * R(1.0 / 403200))));
+    a = dt * sinc;
+    bb = -dt * g;
+    da = dt * dt * dsinc;
+    db = -dt * dt * dg;
+  } else {
+    const R sh = sin_(R(0.5) * x);
+    const R omc = R(2) * sh * sh;   // 1 - cos x
+    const R w2 = w * w;
+    a = s / w;
+    bb = -omc / w;
+    da = (x * c - s) / w2;
+    db = -(x * s - omc) / w2;
+  }
+}
+
 // f(x) and, when JAC, its Jacobian J (NOUT x NIN).
 template <typename R, int NIN, int NOUT, bool JAC, typename S>
 PSM_DEV void model_eval(const ModelView<R, S>& md, const R (&x)[NIN], R (&y)[NOUT], R (&J)[NOUT][NIN]) {
@@ -105,19 +156,26 @@ PSM_DEV void model_eval(const ModelView<R, S>& md, const R (&x)[NIN], R (&y)[NOU
         // sin(wt)/w -> dt and (cos(wt)-1)/w -> 0 with zero w-derivative.
         const R dt = md.par(0);
         const R w = x[4];
-        const R c = cos_(w * dt);
-        const R s = sin_(w * dt);
-        const bool small = abs_(w) < R(1e-6);
-        const R a = small ? dt : s / w;
-        const R bb = small ? R(0) : (c - R(1)) / w;
+        R c, s, a, bb, da, db;
+        if constexpr (IsSingle<R>::value || is_dual<R>::value) {
+          // FP32, and every tangent type: the tangents of the textbook
+          // quotients (second derivatives) cancel even in FP64
+          ct_terms_single(w, dt, a, bb, da, db, c, s);
+        } else {
+          c = cos_(w * dt);
+          s = sin_(w * dt);
+          const bool small = abs_(w) < R(1e-6);
+          a = small ? dt : s / w;
+          bb = small ? R(0) : (c - R(1)) / w;
+          da = small ? R(0) : (dt * c - a) / w;
+          db = small ? R(0) : (-dt * s - bb) / w;
+        }
         y[0] = x[0] + a * x[2] - bb * x[3];
         y[1] = x[1] + bb * x[2] + a * x[3];
         y[2] = c * x[2] + s * x[3];
         y[3] = -s * x[2] + c * x[3];
         y[4] = x[4];
         if (JAC) {
-          const R da = small ? R(0) : (dt * c - a) / w;
-          const R db = small ? R(0) : (-dt * s - bb) / w;
           const R dc = -dt * s, ds = dt * c;
 #pragma unroll
           for (int i = 0; i < 5; ++i)
@@ -232,6 +290,22 @@ PSM_DEV void linearize_extended(const ModelView<R, S>& md, const R (&m)[NIN], R 
     b[i] = s + md.qm(i);
 #pragma unroll
     for (int j = 0; j < NOUT; ++j) C[i][j] = md.qc(i * NOUT + j);
+  }
+  if constexpr (IsSingle<R>::value) {
+    // f(m) - F m in closed form where the model gives one (in FP32 the
+    // difference above cancels at large |m|): 0 for x -> A x; for the
+    // coordinated turn, f is linear in the position and velocity given w, so
+    // f(m) - F m = -F[:, 4] w (and 0 in the w row)
+    if (md.kind == PSM_MODEL_LINEAR) {
+#pragma unroll
+      for (int i = 0; i < NOUT; ++i) b[i] = md.qm(i);
+    } else if (md.kind == PSM_MODEL_COORDINATED_TURN) {
+      if constexpr (NIN == 5 && NOUT == 5) {
+#pragma unroll
+        for (int i = 0; i < 4; ++i) b[i] = md.qm(i) - F[i][4] * m[4];
+        b[4] = md.qm(4);
+      }
+    }
   }
   // conditional moments (parsmooth/linearization/_extended.py:43-56): the
   // conditional factor at the mean replaces the noise
@@ -445,6 +519,211 @@ PSM_DEV void linearize_sigma(const ModelView<R, S>& md, const SigmaView<R>& sp, 
         C[j][i] = s;
       }
   }
+}
+
+// ----------------------------------------------------------------------------
+// Increment f(m + d) - f(m) of a model and its value at the mean, for the
+// single-precision regression below.  For x -> A x and the coordinated turn
+// (linear in position and velocity given w) the increment is formed without
+// m + d: A d, and F(w') d + (F(w') - F(w)) m with w' = w + d_w, whose terms
+// are all of the size of d or of the velocity times a difference of turn-rate
+// terms.  In FP32 a large position (|m| ~ 1e3) rounds m + d and f(m + d) by
+// ~1e-4, above the process noise's chol Q (~6e-5 on the bearings-only model),
+// so forming f(m + d) - f(m) from the two values would leave that rounding in
+// the regression's noise factor.  Other models: f(m + d) - f(m) as
+// evaluated; a tabulated model's values are relative to its first point.
+// ----------------------------------------------------------------------------
+template <typename R, int NIN, int NOUT, typename S>
+PSM_DEV void model_at_mean(const ModelView<R, S>& md, const R (&m)[NIN], R (&f0)[NOUT]) {
+  ModelView<R, S> m0 = md;
+  m0.pt = 0;   // tabulated rows: the first point is the reference
+  model_value<R, NIN, NOUT>(m0, m, f0);
+}
+
+template <typename R, int NIN, int NOUT, typename S>
+PSM_DEV void model_increment(const ModelView<R, S>& md, const R (&m)[NIN], const R (&d)[NIN],
+                             const R (&f0)[NOUT], R (&df)[NOUT]) {
+  if (md.kind == PSM_MODEL_LINEAR) {
+#pragma unroll
+    for (int i = 0; i < NOUT; ++i) {
+      R acc = R(0);
+#pragma unroll
+      for (int j = 0; j < NIN; ++j) acc = fma(md.par(i * NIN + j), d[j], acc);
+      df[i] = acc;
+    }
+    return;
+  }
+  if constexpr (NIN == 5 && NOUT == 5) {
+    if (md.kind == PSM_MODEL_COORDINATED_TURN) {
+      const R dt = md.par(0);
+      R a0, b0, c0, s0, a1, b1, c1, s1, u, v;
+      ct_terms_single(m[4], dt, a0, b0, u, v, c0, s0);
+      ct_terms_single(m[4] + d[4], dt, a1, b1, u, v, c1, s1);
+      const R da = a1 - a0, db = b1 - b0, dc = c1 - c0, ds = s1 - s0;
+      df[0] = d[0] + a1 * d[2] - b1 * d[3] + (da * m[2] - db * m[3]);
+      df[1] = d[1] + b1 * d[2] + a1 * d[3] + (db * m[2] + da * m[3]);
+      df[2] = c1 * d[2] + s1 * d[3] + (dc * m[2] + ds * m[3]);
+      df[3] = -s1 * d[2] + c1 * d[3] + (-ds * m[2] + dc * m[3]);
+      df[4] = d[4];
+      return;
+    }
+  }
+  R x[NIN], f[NOUT];
+#pragma unroll
+  for (int i = 0; i < NIN; ++i) x[i] = m[i] + d[i];
+  model_value<R, NIN, NOUT>(md, x, f);
+#pragma unroll
+  for (int i = 0; i < NOUT; ++i) df[i] = f[i] - f0[i];
+}
+
+// ----------------------------------------------------------------------------
+// Statistical linear regression from centred residuals (single precision).
+// Same F and b as linearize_sigma; the noise factor from the residuals
+// r_p = (f_p - m_f) - F (chi_p - m) of the regression: for a rule whose
+// weights wc are non-negative and whose points reproduce the mean and
+// covariance of x (sum wc (chi - m) = 0, sum wc (chi - m)(chi - m)^T = L L^T:
+// cubature, Gauss-Hermite, unscented), sum wc r r^T = Phi - F P F^T, so
+//   C = tria([sqrt(wc_p) r_p ... | chol Q])            (square-root form),
+//   C = sum wc r r^T + Q, symmetrised                   (covariance form),
+// equal to the reference's chol(Phi + Q - F P F^T) and Phi - F P F^T + Q
+// (parsmooth/linearization/_sigma_points.py:63-85) without forming the
+// difference Phi - F P F^T, which cancels when Q is small against the
+// sigma-point spread.  The points enter as displacements d_p = L xi_p and
+// increments f(m + d_p) - f(m) (model_increment), so neither the difference
+// Phi - F P F^T nor the rounding of a large mean reaches the noise factor.
+// Three passes over the points (mean, cross-covariance, residual columns)
+// re-evaluate the increments instead of keeping npts values per lane.
+// Returns false, with nothing written, when some wc is negative.
+// ----------------------------------------------------------------------------
+template <typename R, int NIN, int NOUT, bool SQRT, typename S>
+PSM_DEV bool linearize_sigma_centred(const ModelView<R, S>& md, const SigmaView<R>& sp, const R (&m)[NIN],
+                                     const R (&L)[NIN][NIN], R (&F)[NOUT][NIN], R (&C)[NOUT][NOUT],
+                                     R (&b)[NOUT]) {
+  for (int p = 0; p < sp.npts; ++p)
+    if (sp.wc[p] < 0.0) return false;
+  R f0[NOUT];
+  model_at_mean<R, NIN, NOUT>(md, m, f0);
+  // point p: displacement d = L xi_p and increment f = f(m + d) - f(m)
+  auto point = [&](int p, R (&d)[NIN], R (&f)[NOUT]) {
+#pragma unroll
+    for (int i = 0; i < NIN; ++i) {
+      R s = R(0);
+#pragma unroll
+      for (int j = 0; j < NIN; ++j) s = fma(L[i][j], sp.pxi(p * NIN + j), s);
+      d[i] = s;
+    }
+    ModelView<R, S> mp = md;
+    mp.pt = p;
+    model_increment<R, NIN, NOUT>(mp, m, d, f0, f);
+  };
+  R mf[NOUT];   // weighted mean of the increments
+  set_zero(mf);
+  for (int p = 0; p < sp.npts; ++p) {
+    R d[NIN], f[NOUT];
+    point(p, d, f);
+    const R w = sp.pwm(p);
+#pragma unroll
+    for (int i = 0; i < NOUT; ++i) mf[i] = fma(w, f[i], mf[i]);
+  }
+  R Psi[NIN][NOUT];
+  set_zero(Psi);
+  for (int p = 0; p < sp.npts; ++p) {
+    R d[NIN], f[NOUT];
+    point(p, d, f);
+    const R w = sp.pwc(p);
+#pragma unroll
+    for (int i = 0; i < NIN; ++i) {
+      const R wx = w * d[i];
+#pragma unroll
+      for (int j = 0; j < NOUT; ++j) Psi[i][j] = fma(wx, f[j] - mf[j], Psi[i][j]);
+    }
+  }
+  // F = (L^-T L^-1 Psi)^T
+  {
+    R X[NIN][NOUT], Y[NIN][NOUT];
+    solve_lower<R, NIN, NOUT>(L, Psi, X);
+    solve_lower_T<R, NIN, NOUT>(L, X, Y);
+#pragma unroll
+    for (int i = 0; i < NOUT; ++i)
+#pragma unroll
+      for (int j = 0; j < NIN; ++j) F[i][j] = Y[j][i];
+  }
+#pragma unroll
+  for (int i = 0; i < NOUT; ++i) {
+    R s = f0[i];
+#pragma unroll
+    for (int j = 0; j < NIN; ++j) s = fma(-F[i][j], m[j], s);
+    b[i] = (s + mf[i]) + md.qm(i);
+  }
+  // residual of point p
+  auto residual = [&](int p, R (&r)[NOUT]) {
+    R d[NIN], f[NOUT];
+    point(p, d, f);
+#pragma unroll
+    for (int i = 0; i < NOUT; ++i) {
+      R s = f[i] - mf[i];
+#pragma unroll
+      for (int j = 0; j < NIN; ++j) s = fma(-F[i][j], d[j], s);
+      r[i] = s;
+    }
+  };
+  if constexpr (SQRT) {
+    R Acc[NOUT][NOUT];
+#pragma unroll
+    for (int i = 0; i < NOUT; ++i)
+#pragma unroll
+      for (int j = 0; j < NOUT; ++j) Acc[i][j] = md.qc(i * NOUT + j);
+    tria_inplace<R, NOUT, NOUT>(Acc);
+    for (int p0 = 0; p0 < sp.npts; p0 += kSigmaBatch) {
+      R Ab[NOUT][NOUT + kSigmaBatch];
+#pragma unroll
+      for (int i = 0; i < NOUT; ++i)
+#pragma unroll
+        for (int j = 0; j < NOUT; ++j) Ab[i][j] = Acc[i][j];
+#pragma unroll
+      for (int k = 0; k < kSigmaBatch; ++k) {
+        R r[NOUT];
+        set_zero(r);
+        const bool live = p0 + k < sp.npts;
+        if (live) residual(p0 + k, r);
+        const R sw = live ? sqrt_(sp.pwc(p0 + k)) : R(0);
+#pragma unroll
+        for (int i = 0; i < NOUT; ++i) Ab[i][NOUT + k] = sw * r[i];
+      }
+      tria_inplace<R, NOUT, NOUT + kSigmaBatch>(Ab);
+#pragma unroll
+      for (int i = 0; i < NOUT; ++i)
+#pragma unroll
+        for (int j = 0; j < NOUT; ++j) Acc[i][j] = Ab[i][j];
+    }
+#pragma unroll
+    for (int i = 0; i < NOUT; ++i)
+#pragma unroll
+      for (int j = 0; j < NOUT; ++j) C[i][j] = Acc[i][j];
+  } else {
+#pragma unroll
+    for (int i = 0; i < NOUT; ++i)
+#pragma unroll
+      for (int j = 0; j < NOUT; ++j) C[i][j] = md.qc(i * NOUT + j);
+    for (int p = 0; p < sp.npts; ++p) {
+      R r[NOUT];
+      residual(p, r);
+      const R w = sp.pwc(p);
+#pragma unroll
+      for (int i = 0; i < NOUT; ++i)
+#pragma unroll
+        for (int j = 0; j < NOUT; ++j) C[i][j] = fma(w * r[i], r[j], C[i][j]);
+    }
+#pragma unroll
+    for (int i = 0; i < NOUT; ++i)
+#pragma unroll
+      for (int j = 0; j < i; ++j) {
+        const R s = R(0.5) * (C[i][j] + C[j][i]);
+        C[i][j] = s;
+        C[j][i] = s;
+      }
+  }
+  return true;
 }
 
 }  // namespace psm

@@ -93,7 +93,7 @@ __global__ __launch_bounds__(kLaneWG) void k_seq_filter(
       copy(L, rL);
     }
     tr.row = i - 1;
-    linearize_point_w<R, NX, NX, SQRT>(tr, sp, use_sigma, rm, rL, st_.F, st_.cq, st_.bt);
+    linearize_point_w<R, NX, NX, SQRT>(tr, sp, use_sigma, rm, rL, st_.F, st_.cq, st_.bt, fb);
     if (nom_m != nullptr) {
       ld_aos(nom_m, i, rm);
       ld_aos(nom_L, i, rL);
@@ -101,7 +101,7 @@ __global__ __launch_bounds__(kLaneWG) void k_seq_filter(
       seq_predict<R, NX, SQRT>(st_.F, st_.cq, st_.bt, m, L, rm, rL);
     }
     ob.row = i;
-    linearize_point_w<R, NX, NY, SQRT>(ob, sp, use_sigma, rm, rL, st_.H, st_.cr, st_.c);
+    linearize_point_w<R, NX, NY, SQRT>(ob, sp, use_sigma, rm, rL, st_.H, st_.cr, st_.c, fb);
     R y[NY];
     ld_aos(ys, i - s - 1, y);
     acc += kf_step_fb<R, NX, NY, SQRT>(st_, y, m, L, fb);
@@ -139,7 +139,7 @@ __global__ __launch_bounds__(kLaneWG) void k_seq_smoother(
     }
     R F[NX][NX], cq[NX][NX], bt[NX];
     tr.row = i;
-    linearize_point_w<R, NX, NX, SQRT>(tr, sp, use_sigma, rm, rL, F, cq, bt);
+    linearize_point_w<R, NX, NX, SQRT>(tr, sp, use_sigma, rm, rL, F, cq, bt, fb);
     R xm[NX], xL[NX][NX];
     ld_aos(fm, i, xm);
     ld_aos(fL, i, xL);

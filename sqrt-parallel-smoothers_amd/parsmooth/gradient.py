@@ -13,7 +13,21 @@ the carry of ``_rev_iter`` runs while i < N + 1), and
 theta_bar = (d f / d theta)^T u + d ell / d theta; the initial nominal gets no
 gradient (:125).
 
-Here the same quantity is evaluated in forward mode, one parameter direction
+Two evaluations of that same quantity are provided.
+
+Reverse mode (``value_and_grad``, the default): the reference's reverse sum
+itself, through the hand-written adjoint of one filter + smoother pass
+(psm_adjoint_prepare / psm_filter_vjp / psm_filter_smoother_vjp):
+
+    x_bar, theta_ell = d ell / d (x*, theta)           (psm_filter_vjp, Fisher's identity)
+    u <- x_bar + J_x^T u, N + 1 times from u = x_bar  (psm_filter_smoother_vjp)
+    theta_bar = (d f / d theta)^T u + theta_ell        (one more adjoint pass)
+
+N + 3 adjoint passes at x* whatever the number of parameters; the primal at
+x* is checkpointed once (filtered / smoothed covariances, gains, filter-element
+matrices) and every adjoint pass reads it.
+
+Forward mode (``value_and_grad(..., mode="forward")``): one parameter direction
 e_p at a time, through the tangent (dual-number) instantiation of the same
 kernels:
 
@@ -21,10 +35,11 @@ kernels:
     dx    <- v + J_x dx, N + 1 times from dx = v (tangent passes at x*)
     d ell  = (d ell / d x*) dx + (d ell / d theta) e_p   (one tangent filter pass)
 
-which is the transpose of the reverse sum, so the gradient is identical to the
-reference's.  The parameterisation theta -> model arrays is a small torch
-function evaluated on the host; its Jacobian (torch.func.jacfwd) provides the
-tangents of the model parameters, noise factors and prior.
+which is the transpose of the reverse sum: P (N + 3) tangent passes.
+
+The parameterisation theta -> model arrays is a small torch function
+evaluated on the host; its Jacobian (torch.func.jacfwd) maps the model-array
+cotangents (reverse) or supplies their tangents (forward).
 """
 from __future__ import annotations
 
@@ -188,8 +203,14 @@ class IteratedLoglikelihood:
         out = self._tangent_pass(models, sigma, nom, dnom)
         return ((out[0][..., 0], out[1][..., 0]), (out[0][..., 1], out[1][..., 1]))
 
-    def value_and_grad(self, theta: torch.Tensor):
-        """(ell (n_seq,), grad (n_seq, P)) with the reference's implicit gradient."""
+    def value_and_grad(self, theta: torch.Tensor, mode: str = "reverse"):
+        """(ell (n_seq,), grad (n_seq, P)) with the reference's implicit gradient,
+        by the reverse-mode adjoint (default) or by forward-mode tangents
+        (``mode="forward"``)."""
+        if mode == "reverse":
+            return self._value_and_grad_reverse(theta)
+        if mode != "forward":
+            raise ValueError(f"mode must be 'reverse' or 'forward', not {mode!r}")
         theta = theta.detach().to(torch.float64).cpu()
         ell, xm, xL = self.value(theta)
         vals = self.build(theta)
@@ -211,6 +232,150 @@ class IteratedLoglikelihood:
                              (_dual(xm, dm), _dual(xL, dL)), filt, ell_d)
             grads[:, p] = ell_d[:, 1]
         return ell, grads
+
+
+    # -------------------------------------------------------------- reverse mode
+    def _adjoint_problem(self):
+        """The problem the adjoint passes run on: this one in FP64; for FP32 an
+        FP64 twin over the same FP32-rounded observations and model arrays.
+        The adjoint is FP64 either way, but its checkpoint (filtered and
+        smoothed moments at x*) comes from the primal pass of the problem it is
+        given, and FP32 smoothed means carry an absolute rounding of ~1e-7 of
+        the track's scale into every step's residual (y - c - H m): summed over
+        16,384 steps that moves the sensor-position gradients by ~5e-3 against
+        central differences, where the FP64 checkpoint of the same FP32 fixed
+        point stays within 1e-5."""
+        if self.dtype == torch.float64:
+            return self.prob, self.ys
+        twin = getattr(self, "_twin", None)
+        if twin is None:
+            p = self.prob
+            twin = (Problem(self.nx, self.ny, p.lengths, torch.float64, True, self.dev), self.ys.double())
+            self._twin = twin
+        return twin
+
+    def _adjoint_models(self, vals):
+        """Model arrays of the adjoint problem: the FP32-rounded arrays of the
+        primal, in FP64, for an FP32 objective."""
+        if self.dtype == torch.float64:
+            return _ModelSet(self.kinds, (self.nx, self.ny), vals, self.dev, torch.float64)
+        rounded = {k: to_dev(v, self.dtype, self.dev).double() for k, v in vals.items()}
+        return _ModelSet(self.kinds, (self.nx, self.ny), rounded, self.dev, torch.float64)
+
+    def _adjoint_workspace(self, prob):
+        nbytes = int(N.lib().psm_adjoint_workspace_bytes(ctypes.byref(prob.struct)))
+        if nbytes == 0:
+            N.check(-1, "psm_adjoint_workspace_bytes")
+        ws = getattr(self, "_aws", None)
+        if ws is None or ws.numel() < nbytes:
+            ws = torch.empty(nbytes, dtype=torch.uint8, device=self.dev)
+            self._aws = ws
+        return ws
+
+    def _model_cotangents(self, th: torch.Tensor, vals, jac) -> torch.Tensor:
+        """Per-sequence parameter gradients from the adjoint's model-array
+        cotangents th (n_seq, NTH; layout of PSM_ADJ_NTH in include/psm.h)
+        and the Jacobian of the parameterisation."""
+        nx, ny = self.nx, self.ny
+        n_seq = th.shape[0]
+        P = next(iter(jac.values())).shape[-1]
+        sizes = (("trans_params", nx * nx), ("trans_q_mean", nx), ("trans_q_chol", nx * nx),
+                 ("obs_params", ny * nx), ("obs_q_mean", ny), ("obs_q_chol", ny * ny), ("m0", nx), ("L0", nx * nx))
+        off, o = {}, 0
+        for key, k in sizes:
+            off[key] = o
+            o += k
+        th = th.double().cpu()
+        grad = torch.zeros(n_seq, P, dtype=torch.float64)
+        for key in KEYS:
+            v = torch.as_tensor(vals[key])
+            J = jac[key].to(torch.float64)
+            per_seq = key in ("m0", "L0") and v.dim() == (2 if key == "m0" else 3)
+            k = (v[0] if per_seq else v).numel()
+            cot = th[:, off[key]:off[key] + k]                      # (n_seq, k)
+            if per_seq:
+                grad += torch.einsum("sk,skp->sp", cot, J.reshape(n_seq, k, P))
+            else:
+                grad += cot @ J.reshape(k, P)
+        return grad
+
+    def _value_and_grad_reverse(self, theta: torch.Tensor):
+        theta = theta.detach().to(torch.float64).cpu()
+        ell, xm, xL = self.value(theta)
+        vals = self.build(theta)
+        jac = torch.func.jacfwd(self.build)(theta)
+        p, ys = self._adjoint_problem()
+        models = self._adjoint_models(vals)
+        sigma, _keep = self.method.sigma(self.nx, torch.float64, self.dev)
+        sref = ctypes.byref(sigma) if sigma is not None else None
+        m0, L0 = models.keep["m0"], models.keep["L0"]
+        aws = self._adjoint_workspace(p)
+        f64 = dict(dtype=torch.float64, device=self.dev)
+        filt = (torch.empty((p.n_states, self.nx), **f64), torch.empty((p.n_states, self.nx, self.nx), **f64))
+        sm = (torch.empty_like(filt[0]), torch.empty_like(filt[1]))
+        xm, xL = xm.double().contiguous(), xL.double().contiguous()
+        base = (ctypes.byref(p.struct), ctypes.byref(models.trans), ctypes.byref(models.obs), sref,
+                N.ptr(ys), N.ptr(m0), N.ptr(L0), N.ptr(xm), N.ptr(xL))
+        rc = N.lib().psm_adjoint_prepare(*base, N.ptr(filt[0]), N.ptr(filt[1]), N.ptr(sm[0]), N.ptr(sm[1]),
+                                         N.ptr(p.workspace), p.ws_bytes, N.ptr(aws), aws.numel(), N.stream_ptr())
+        N.check(rc, "psm_adjoint_prepare")
+        M, nx, ny = p.n_states, self.nx, self.ny
+        nth = 3 * nx * nx + 2 * nx + ny * nx + ny + ny * ny
+        xb = (torch.empty((M, nx), **f64), torch.empty((M, nx, nx), **f64))
+        th_ell = torch.empty((p.n_seq, nth), **f64)
+        rc = N.lib().psm_filter_vjp(*base, N.ptr(sm[0]), N.ptr(sm[1]), N.ptr(xb[0]), N.ptr(xb[1]), N.ptr(th_ell),
+                                    N.ptr(aws), aws.numel(), N.stream_ptr())
+        N.check(rc, "psm_filter_vjp")
+        u = (xb[0].clone(), xb[1].clone())
+        nxt = (torch.empty_like(u[0]), torch.empty_like(u[1]))
+
+        def vjp(cot, add, out, th):
+            rc_ = N.lib().psm_filter_smoother_vjp(*base, N.ptr(sm[0]), N.ptr(sm[1]), N.ptr(cot[0]), N.ptr(cot[1]),
+                                                  N.ptr(add[0] if add else None), N.ptr(add[1] if add else None),
+                                                  N.ptr(out[0]), N.ptr(out[1]), N.ptr(th), N.ptr(aws),
+                                                  aws.numel(), N.stream_ptr())
+            N.check(rc_, "psm_filter_smoother_vjp")
+
+        # u <- x_bar + J_x^T u, N + 1 times (the reference's _rev_iter loop)
+        for _ in range(self.n_iter + 1):
+            vjp(u, xb, nxt, None)
+            u, nxt = nxt, u
+        th_f = torch.empty((p.n_seq, nth), **f64)
+        vjp(u, None, nxt, th_f)
+        return ell, self._model_cotangents(th_ell + th_f, vals, jac).to(self.dev)
+
+    def vjp_pass(self, theta, nominal, cot_mean, cot_chol):
+        """Vector-Jacobian product of one filter-smoother pass at ``nominal``:
+        for cotangents of its smoothed (mean, chol), returns the cotangents of
+        the nominal (mean, chol) and the per-sequence parameter gradient
+        (n_seq, P)."""
+        theta = torch.as_tensor(theta, dtype=torch.float64).detach().cpu()
+        vals = self.build(theta)
+        jac = torch.func.jacfwd(self.build)(theta)
+        p, ys = self._adjoint_problem()
+        models = self._adjoint_models(vals)
+        sigma, _keep = self.method.sigma(self.nx, torch.float64, self.dev)
+        sref = ctypes.byref(sigma) if sigma is not None else None
+        aws = self._adjoint_workspace(p)
+        nm = tuple(to_dev(a, self.dtype, self.dev).double().contiguous() for a in nominal)
+        f64 = dict(dtype=torch.float64, device=self.dev)
+        filt = (torch.empty((p.n_states, self.nx), **f64), torch.empty((p.n_states, self.nx, self.nx), **f64))
+        sm = (torch.empty_like(filt[0]), torch.empty_like(filt[1]))
+        base = (ctypes.byref(p.struct), ctypes.byref(models.trans), ctypes.byref(models.obs), sref,
+                N.ptr(ys), N.ptr(models.keep["m0"]), N.ptr(models.keep["L0"]), N.ptr(nm[0]), N.ptr(nm[1]))
+        rc = N.lib().psm_adjoint_prepare(*base, N.ptr(filt[0]), N.ptr(filt[1]), N.ptr(sm[0]), N.ptr(sm[1]),
+                                         N.ptr(p.workspace), p.ws_bytes, N.ptr(aws), aws.numel(), N.stream_ptr())
+        N.check(rc, "psm_adjoint_prepare")
+        cot = (to_dev(cot_mean, torch.float64, self.dev).contiguous(),
+               to_dev(cot_chol, torch.float64, self.dev).contiguous())
+        out = (torch.empty((p.n_states, self.nx), **f64), torch.empty((p.n_states, self.nx, self.nx), **f64))
+        nth = 3 * self.nx ** 2 + 2 * self.nx + self.ny * self.nx + self.ny + self.ny ** 2
+        th = torch.empty((p.n_seq, nth), **f64)
+        rc = N.lib().psm_filter_smoother_vjp(*base, N.ptr(sm[0]), N.ptr(sm[1]), N.ptr(cot[0]), N.ptr(cot[1]),
+                                             None, None, N.ptr(out[0]), N.ptr(out[1]), N.ptr(th), N.ptr(aws),
+                                             aws.numel(), N.stream_ptr())
+        N.check(rc, "psm_filter_smoother_vjp")
+        return (out[0], out[1]), self._model_cotangents(th, vals, jac)
 
 
 def bearings_parameterisation(dt: float, m0, L0):

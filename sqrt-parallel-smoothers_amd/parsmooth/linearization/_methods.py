@@ -56,7 +56,10 @@ _METHODS = {}
 
 def resolve(method) -> Method:
     """Map a linearization callable (or a functools.partial of one with keyword
-    arguments such as ``order=5``) to its device implementation."""
+    arguments such as ``order=5``) to its device implementation.  A resolved
+    method is returned as it is."""
+    if isinstance(method, Method):
+        return method
     kwargs = {}
     fn = method
     while isinstance(fn, functools.partial):

@@ -19,6 +19,7 @@ criterion runs on the host between passes, each pass one call of
 """
 from __future__ import annotations
 
+import dis
 from collections import OrderedDict
 from typing import Callable, Optional, Union
 
@@ -71,50 +72,50 @@ class FixedIterations:
         return i < self.n
 
 
-class _Opaque:
-    """Stands in for a trajectory while a criterion is probed: any use raises."""
-
-    def __getattr__(self, name):
-        raise _NotFixed
-
-    def __getitem__(self, key):
-        raise _NotFixed
-
-    def __iter__(self):
-        raise _NotFixed
-
-    def __array__(self, *a, **k):
-        raise _NotFixed
-
-
-class _NotFixed(Exception):
-    pass
-
-
-_PROBE_LIMIT = 1 << 16
+_SKIP_OPS = frozenset(("RESUME", "NOP", "CACHE", "COPY_FREE_VARS", "MAKE_CELL", "PRECALL"))
 
 
 def _fixed_count(criterion) -> Optional[int]:
     """The number of fixed-point applications a criterion allows when it
-    depends on the iteration count only, else None.  The criterion is called
-    with i = 1, 2, ... and stand-in trajectories that raise on any use; it is
-    a count criterion when it returns a bool each time and first returns False
-    at some i <= 65536 (that i is the number of applications)."""
+    depends on the iteration count only, else None.  Recognised without
+    calling it (a criterion may keep state): ``FixedIterations(n)``, and a
+    function whose whole body is ``i < n`` or ``i <= n`` on its first
+    argument, with ``n`` an int constant, closure variable or global."""
     if isinstance(criterion, FixedIterations):
         return max(1, criterion.n)
-    if criterion is _default_criterion:
+    fn = criterion
+    code = getattr(fn, "__code__", None)
+    if code is None or code.co_argcount < 1:
         return None
-    opaque = _Opaque()
     try:
-        for i in range(1, _PROBE_LIMIT + 1):
-            r = criterion(i, opaque, opaque)
-            if not isinstance(r, bool):
-                return None
-            if not r:
-                return i
-    except Exception:
+        ins = [x for x in dis.get_instructions(fn) if x.opname not in _SKIP_OPS]
+    except TypeError:
         return None
-    return None
+    if len(ins) != 4:
+        return None
+    first, bound, cmp, ret = ins
+    if first.opname != "LOAD_FAST" or first.argval != code.co_varnames[0]:
+        return None
+    if cmp.opname != "COMPARE_OP" or ret.opname != "RETURN_VALUE":
+        return None
+    op = str(cmp.argrepr).replace("bool(", "").rstrip(")")
+    if op not in ("<", "<="):
+        return None
+    if bound.opname == "LOAD_CONST":
+        n = bound.argval
+    elif bound.opname == "LOAD_DEREF" and bound.argval in code.co_freevars and fn.__closure__:
+        try:
+            n = fn.__closure__[code.co_freevars.index(bound.argval)].cell_contents
+        except ValueError:   # empty cell
+            return None
+    elif bound.opname == "LOAD_GLOBAL" and bound.argval in fn.__globals__:
+        n = fn.__globals__[bound.argval]
+    else:
+        return None
+    if isinstance(n, bool) or not isinstance(n, int):
+        return None
+    # the carry starts after one application; criterion(i) is asked with i = 1, 2, ...
+    return max(1, n if op == "<" else n + 1)
 
 
 def _default_criterion(_i, nominal_traj_prev, curr_nominal_traj):

@@ -172,15 +172,29 @@ def test_are_inputs_compatible_follows_the_reference():
 
 def test_fixed_count_criterion_probe():
     """iterated_smoothing recognises criteria that depend on the iteration count
-    only (run as one launch sequence) and leaves every other criterion to the
-    host loop."""
+    only (run as one launch sequence) without calling them, and leaves every
+    other criterion to the host loop."""
     from parsmooth.methods import FixedIterations, _default_criterion, _fixed_count
+    n_iter, calls = 6, []
+
+    def counted(i, *_):
+        return i < n_iter
+
+    def stateful(i, prev, cur):
+        calls.append(i)
+        return i < 3
+
     assert _fixed_count(lambda i, *_: i < 10) == 10
     assert _fixed_count(lambda i, prev, cur: i < 3) == 3
+    assert _fixed_count(lambda i, *_: i < n_iter) == 6          # closure variable
+    assert _fixed_count(lambda i, *_: i <= 4) == 5
+    assert _fixed_count(counted) == 6
+    assert _fixed_count(lambda i, *_: i < 0) == 1               # the first application always runs
     assert _fixed_count(FixedIterations(7)) == 7
-    assert _fixed_count(lambda i, *_: False) == 1
+    assert _fixed_count(stateful) is None and calls == []       # never called
     assert _fixed_count(_default_criterion) is None
+    assert _fixed_count(lambda i, *_: False) is None
     assert _fixed_count(lambda i, prev, cur: i < 5 and prev.mean is not None) is None
     assert _fixed_count(lambda i, prev, cur: bool((prev[0] - cur[0]).sum() > 0)) is None
-    assert _fixed_count(lambda i, *_: True) is None          # never stops within the probe limit
-    assert _fixed_count(lambda i, *_: 1 if i < 3 else 0) is None   # not a bool
+    assert _fixed_count(lambda i, *_: i < 2.5) is None
+    assert _fixed_count(lambda i, *_: 3 > i) is None

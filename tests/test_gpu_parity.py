@@ -630,13 +630,35 @@ def test_fp32_bearings_vs_fp64_oracle():
     close(npy(got.mean), ref.mean, tol=1e-4, what="fp32 mean")
 
 
+def _ct_turn_rate_column(m, dt):
+    """The turn-rate column of the coordinated-turn Jacobian from the Taylor
+    series of d/dw sin(w dt)/w = dt^2 s'(x) and d/dw (cos(w dt) - 1)/w =
+    -dt^2 g'(x), x = w dt (exact to FP64 rounding for |x| < 1e-2, where the
+    closed forms (x cos x - sin x)/x^2 and (x sin x - 1 + cos x)/x^2 cancel)."""
+    x = m[:, 4] * dt
+    x2 = x * x
+    ds = -x / 3 + x * x2 / 30 - x * x2 * x2 / 840
+    dg = 0.5 - x2 / 8 + x2 * x2 / 144 - x2 * x2 * x2 / 5760
+    da, db = dt * dt * ds, -dt * dt * dg
+    col = np.zeros((m.shape[0], 5))
+    col[:, 0] = da * m[:, 2] - db * m[:, 3]
+    col[:, 1] = db * m[:, 2] + da * m[:, 3]
+    col[:, 2] = -dt * np.sin(x) * m[:, 2] + dt * np.cos(x) * m[:, 3]
+    col[:, 3] = -dt * np.cos(x) * m[:, 2] - dt * np.sin(x) * m[:, 3]
+    col[:, 4] = 1.0
+    return col
+
+
 def test_fp32_extended_small_turn_rate():
     """FP32 extended linearization of the coordinated turn at small turn rates
     (w dt from 1e-4 down to 2e-8, above the model's 1e-6 branch on w): the
     turn-rate column of the Jacobian, e.g. d/dw (1 - cos(w dt)) / w, cancels in
-    FP32 arithmetic (cos(w dt) rounds to 1 and the derivative doubles), so FP32
-    problems linearize in FP64; each entry must match the FP64 oracle to 1e-5 of
-    its own size, and b = f(m) - F m to FP32 rounding of a large |m|."""
+    the closed form (in FP32 cos(w dt) rounds to 1 and the derivative doubles),
+    so FP32 problems evaluate it from its Taylor series.  Each entry must match
+    the exact derivative to 1e-5 of its own size, and the FP64 oracle (the
+    reference's closed form, which itself loses up to ~1e-16 / (w dt)^2 of the
+    value) to 1e-5 plus the oracle's own error; b = f(m) - F m to FP32 rounding
+    of a large |m|."""
     dt = float(np.float32(0.01))
     Q, _, _, trans_f = M.make_parameters(0.01, 0.1, 0.5, dt, (-1.5, 0.5), (1.0, 1.0))
     cQ = np.asarray(np.linalg.cholesky(Q), np.float32).astype(np.float64)
@@ -651,35 +673,50 @@ def test_fp32_extended_small_turn_rate():
     F, _, b = [npy(g) for g in extended(FunctionalModel(trans_f, MVNSqrt(t32(np.zeros(5)), t32(cQ))),
                                         MVNSqrt(t32(m), t32(L)))]
     Fo, _, bo = O.extended(O.FunctionalModel(O.CoordinatedTurn(dt), O.MVNSqrt(np.zeros(5), cQ)), O.MVNSqrt(m, L))
-    col, colo = F[:, :, 4], Fo[:, :, 4]
-    assert np.all(np.abs(col - colo) <= 1e-5 * np.abs(colo) + 1e-12), (col, colo)
-    close(F, Fo, tol=1e-6, what="fp32 CT Jacobian")
+    col, colo, exact = F[:, :, 4], Fo[:, :, 4], _ct_turn_rate_column(m, dt)
+    # the series agrees with the oracle's closed form where that one is accurate (w dt ~ 1e-2)
+    mm = m.copy()
+    mm[:, 4] = np.array([1.0, -2.0, 0.5, -0.7, 1.3, 0.9])
+    assert np.allclose(_ct_turn_rate_column(mm, dt), O.CoordinatedTurn(dt).jac(mm)[:, :, 4], rtol=1e-10, atol=0)
+    assert np.all(np.abs(col - exact) <= 1e-5 * np.abs(exact) + 1e-12), (col, exact)
+    assert np.all(np.abs(col - colo) <= 1e-5 * np.abs(colo) + np.abs(exact - colo) + 1e-12), (col, colo)
+    close(np.delete(F, 4, axis=2), np.delete(Fo, 4, axis=2), tol=1e-6, what="fp32 CT Jacobian")
     close(b, bo, tol=1e-6, what="fp32 CT offset")
 
 
 @pytest.mark.parametrize("method", ["cubature", "unscented", "gauss_hermite"])
-def test_fp32_sigma_regression_small_noise(method):
+@pytest.mark.parametrize("scale", [1.0, 100.0, 1500.0])
+def test_fp32_sigma_regression_small_noise(method, scale):
     """FP32 sigma-point linearization of the coordinated-turn transition, whose
     process noise (chol ~6e-5 on the positions) is tiny against the sigma-point
     spread (0.3): the regression noise factor chol(Phi + Q - F P F^T) against the
-    FP64 oracle on the same FP32-rounded inputs.  The cancellation of Phi and
-    F P F^T is exact only with weights that sum to one and points with
-    sum w xi xi^T = I, which an FP32-rounded sigma table breaks by ~1e-8 -- more
-    than Q itself -- so the table is FP64 and this factor must match."""
+    FP64 oracle on the same FP32-rounded inputs.  The regression works from
+    centred residuals with FP64 weights, so neither the cancellation of Phi and
+    F P F^T nor weights rounded off one (an FP32 table leaves ~1e-8, more than
+    Q itself) reaches the factor; positions up to 1500 (scale) check that the
+    sigma points' increments are formed without rounding m + d (at 1500 an FP32
+    position carries ~1e-4 of rounding, above chol Q)."""
     r32 = lambda a: np.asarray(a, np.float32).astype(np.float64)
     dt = float(np.float32(0.01))
     Q, _, _, trans_f = M.make_parameters(0.01, 0.1, 0.5, dt, (-1.5, 0.5), (1.0, 1.0))
     cQ = r32(np.linalg.cholesky(Q))
     rng = np.random.RandomState(11)
     B = 64
-    m = r32(np.array([0.1, 0.2, 1.0, 0.0, 0.5]) + rng.randn(B, 5))
+    m = np.array([0.1, 0.2, 1.0, 0.0, 0.5]) + rng.randn(B, 5)
+    m[:, :2] *= scale
+    m = r32(m)
     L = r32(np.stack([0.3 * np.eye(5) + 0.05 * np.tril(rng.randn(5, 5), -1) for _ in range(B)]))
     t32 = lambda a: torch.as_tensor(np.asarray(a), dtype=torch.float32, device=DEV)
     got = [npy(g) for g in DMETH[method](FunctionalModel(trans_f, MVNSqrt(t32(np.zeros(5)), t32(cQ))),
                                          MVNSqrt(t32(m), t32(L)))]
     ref = OMETH[method](O.FunctionalModel(O.CoordinatedTurn(dt), O.MVNSqrt(np.zeros(5), cQ)), O.MVNSqrt(m, L))
     close(got[0], ref[0], tol=TOL32, what=f"fp32 {method} F")
-    close(got[2], ref[2], tol=TOL32, what=f"fp32 {method} b")
+    if scale == 1.0:
+        close(got[2], ref[2], tol=TOL32, what=f"fp32 {method} b")
+    # the linearized prediction F m + b (b = f(m) + ... - F m carries the FP32
+    # rounding of F m, ~1e-7 of |m|, which the prediction cancels)
+    pred = lambda F, b: np.einsum("bij,bj->bi", F, m) + b
+    close(pred(got[0], got[2]), pred(ref[0], ref[2]), tol=TOL32, what=f"fp32 {method} F m + b")
     # the noise factor relative to its own scale (Q ~ 1e-9 .. 1e-3)
     Pg, Pr = gram(got[1]), gram(ref[1])
     err = float(np.abs(Pg - Pr).max() / np.abs(Pr).max())
