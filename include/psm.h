@@ -242,6 +242,26 @@ int psm_filter_smoother(const psm_problem_t* prob, const psm_model_t* trans,
                         void* filt_m, void* filt_L, void* out_m, void* out_L, void* workspace,
                         size_t workspace_bytes, int32_t use_graph, void* stream);
 
+/* Size of the library-owned workspace of psm_filter_smoother_noalloc (per GPU). */
+#define PSM_STATIC_WORKSPACE_BYTES (64u << 20)
+
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under hipGraph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): psm_filter_smoother without a caller-supplied workspace
+ * and without any allocation (no hipMalloc / hipFree, no event or graph
+ * creation), eager on the caller's stream.  The scan workspace is a static
+ * device buffer of PSM_STATIC_WORKSPACE_BYTES that each GPU holds until the
+ * process ends; problems whose psm_workspace_bytes exceeds it are rejected
+ * with PSM_E_WORKSPACE (64 MiB holds a single FP64 nx=5 sequence of about
+ * 55,000 steps).  Every call on one GPU shares that buffer: calls that
+ * overlap on one GPU (concurrent streams or threads) race and must be
+ * serialised by the caller.  Nothing in it needs to start at zero. */
+int psm_filter_smoother_noalloc(const psm_problem_t* prob, const psm_model_t* trans,
+                                const psm_model_t* obs, const psm_sigma_t* sigma, const void* ys,
+                                const void* m0, const void* L0, const void* nom_m,
+                                const void* nom_L, void* filt_m, void* filt_L, void* out_m,
+                                void* out_L, void* stream);
+
 /* The whole fixed-count iteration of iterated smoothing as one launch
  * sequence: n_iter filter + smoother passes, pass k at the previous pass's
  * smoothed trajectory (pass 0 at init), then, when ell != NULL, the

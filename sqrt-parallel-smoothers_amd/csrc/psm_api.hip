@@ -262,8 +262,12 @@ static std::mutex g_tmu;
 static bool g_timing = false;
 static std::vector<TimingRec> g_trecs;
 
+// set while psm_filter_smoother_noalloc runs on this thread: no timing events
+// are created then (creating them allocates)
+static thread_local bool g_no_alloc = false;
+
 int timing_begin(const char* name, hipStream_t st) {
-  if (!g_timing) return -1;
+  if (!g_timing || g_no_alloc) return -1;
   hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
   if (hipStreamIsCapturing(st, &cap) == hipSuccess && cap != hipStreamCaptureStatusNone) return -1;
   TimingRec r{name, nullptr, nullptr};
@@ -473,6 +477,72 @@ int psm_iterated_smoother_graph(const psm_problem_t* p, const psm_model_t* trans
     push(key, q);
   key.push_back((uint64_t)wsb);
   return run_as_graph(key, st, run);
+}
+
+// ---------------------------------------------------------------- library-owned workspace
+// The workspace of psm_filter_smoother_noalloc: a static device array that each
+// GPU holds from the library's load until the process ends.  Host code never
+// uses its name as an address: the device address of each GPU's copy is taken
+// once per device with hipGetSymbolAddress and kept in g_scratch_ptr.
+static __device__ __align__(256) unsigned char scratch[PSM_STATIC_WORKSPACE_BYTES];
+constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_mu;
+static void* g_scratch_ptr[kMaxDevices] = {};
+
+static int scratch_of_current_device(void** out) {
+  int dev = -1;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess || dev < 0 || dev >= kMaxDevices) {
+    set_error(std::string("psm_filter_smoother_noalloc: no usable current device (") +
+              hipGetErrorString(e) + ")");
+    return PSM_E_HIP;
+  }
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (g_scratch_ptr[dev] == nullptr) {
+    void* ptr = nullptr;
+    e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || ptr == nullptr) {
+      set_error(std::string("psm_filter_smoother_noalloc: hipGetSymbolAddress: ") + hipGetErrorString(e));
+      return PSM_E_HIP;
+    }
+    g_scratch_ptr[dev] = ptr;
+  }
+  *out = g_scratch_ptr[dev];
+  return PSM_OK;
+}
+
+int psm_filter_smoother_noalloc(const psm_problem_t* p, const psm_model_t* trans,
+                                const psm_model_t* obs, const psm_sigma_t* sigma, const void* ys,
+                                const void* m0, const void* L0, const void* nom_m,
+                                const void* nom_L, void* filt_m, void* filt_L, void* out_m,
+                                void* out_L, void* stream) {
+  int rc = check_problem(p);
+  if (rc) return rc;
+  if ((rc = check_model(trans, p->nx, p->nx))) return rc;
+  if ((rc = check_model(obs, p->nx, p->ny))) return rc;
+  const OpsTable* t = need(p->dtype, p->form, p->nx, p->ny);
+  if (!t) return PSM_E_UNSUPPORTED;
+  const size_t need_bytes = t->workspace_bytes(p);
+  if (need_bytes > (size_t)PSM_STATIC_WORKSPACE_BYTES) {
+    set_error("psm_filter_smoother_noalloc: this problem needs " + std::to_string(need_bytes) +
+              " bytes of workspace; the library's static workspace holds " +
+              std::to_string((size_t)PSM_STATIC_WORKSPACE_BYTES) +
+              " (use psm_filter_smoother with a caller-supplied workspace)");
+    return PSM_E_WORKSPACE;
+  }
+  void* ws = nullptr;
+  if ((rc = scratch_of_current_device(&ws))) return rc;
+  // every kernel of the pass writes each workspace word before reading it, so
+  // nothing there needs clearing between calls
+  g_no_alloc = true;
+  hipStream_t st = (hipStream_t)stream;
+  rc = t->filter(p, trans, obs, sigma, ys, m0, L0, nom_m, nom_L, filt_m, filt_L, nullptr, ws,
+                 PSM_STATIC_WORKSPACE_BYTES, st, 0);
+  if (rc == PSM_OK)
+    rc = t->smoother(p, trans, sigma, filt_m, filt_L, nom_m, nom_L, out_m, out_L, ws,
+                     PSM_STATIC_WORKSPACE_BYTES, st, 1);
+  g_no_alloc = false;
+  return rc;
 }
 
 // ---------------------------------------------------------------- reverse-mode adjoint

@@ -39,7 +39,7 @@ EXPORTS = (
     "psm_filter", "psm_smoother", "psm_filter_smoother", "psm_mean_sq_diff", "psm_clear_graphs",
     "psm_timing_enable", "psm_timing_report", "psm_chunk_size", "psm_autotune",
     "psm_sample_workspace_bytes", "psm_sample", "psm_seq_filter", "psm_seq_smoother",
-    "psm_iterated_smoother_graph", "psm_adjoint_workspace_bytes",
+    "psm_iterated_smoother_graph", "psm_filter_smoother_noalloc", "psm_adjoint_workspace_bytes",
     "psm_adjoint_prepare", "psm_filter_smoother_vjp", "psm_filter_vjp",
 )
 
@@ -91,6 +91,9 @@ _SIGS = {
     "psm_iterated_smoother_graph": (ctypes.c_int, [ctypes.POINTER(PsmProblem), ctypes.POINTER(PsmModel),
                                                    ctypes.POINTER(PsmModel), ctypes.POINTER(PsmSigma)]
                                     + [_P] * 5 + [_I32] + [_P] * 7 + [_P, ctypes.c_size_t, _I32, _P]),
+    "psm_filter_smoother_noalloc": (ctypes.c_int, [ctypes.POINTER(PsmProblem), ctypes.POINTER(PsmModel),
+                                                   ctypes.POINTER(PsmModel), ctypes.POINTER(PsmSigma)]
+                                    + [_P] * 10),
     "psm_adjoint_workspace_bytes": (ctypes.c_size_t, [ctypes.POINTER(PsmProblem)]),
     "psm_adjoint_prepare": (ctypes.c_int, [ctypes.POINTER(PsmProblem), ctypes.POINTER(PsmModel),
                                            ctypes.POINTER(PsmModel), ctypes.POINTER(PsmSigma)]

@@ -198,3 +198,22 @@ def test_fixed_count_criterion_probe():
     assert _fixed_count(lambda i, prev, cur: bool((prev[0] - cur[0]).sum() > 0)) is None
     assert _fixed_count(lambda i, *_: i < 2.5) is None
     assert _fixed_count(lambda i, *_: 3 > i) is None
+
+
+def test_noalloc_refuses_problems_beyond_the_static_workspace():
+    """psm_filter_smoother_noalloc rejects a problem whose workspace exceeds
+    PSM_STATIC_WORKSPACE_BYTES with PSM_E_WORKSPACE before touching a device
+    (here: a single FP64 sequence of 2^18 steps; the pointers are never read)."""
+    lib = N.load_library()
+    txt = open(os.path.join(ROOT, "include", "psm.h")).read()
+    m = re.search(r"#define PSM_STATIC_WORKSPACE_BYTES \((\d+)u << (\d+)\)", txt)
+    limit = int(m.group(1)) << int(m.group(2))
+    dummy = ctypes.c_void_p(0x1000)
+    prob = N.PsmProblem(N.PSM_F64, N.FORM_SQRT, 5, 2, 1, 0, (1 << 18) + 1, dummy, 0.0, 0, None)
+    assert lib.psm_workspace_bytes(ctypes.byref(prob)) > limit
+    trans = N.PsmModel(N.MODEL_COORDINATED_TURN, 5, 5, 0, dummy, dummy, dummy)
+    obs = N.PsmModel(N.MODEL_BEARINGS, 5, 2, 0, dummy, dummy, dummy)
+    rc = lib.psm_filter_smoother_noalloc(ctypes.byref(prob), ctypes.byref(trans), ctypes.byref(obs), None,
+                                         *([dummy] * 9), None)
+    assert rc == -4
+    assert b"static workspace" in lib.psm_last_error()
