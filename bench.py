@@ -177,14 +177,14 @@ def build_grad_objective(n_seq, T, n_iter, dtype, seed=5):
     return obj, theta
 
 
-def time_gradient(obj, theta, steps, warmup):
+def time_gradient(obj, theta, steps, warmup, mode="reverse"):
     import torch
     for _ in range(warmup):
-        obj.value_and_grad(theta)
+        obj.value_and_grad(theta, mode=mode)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for _ in range(steps):
-        ell, grad = obj.value_and_grad(theta)
+        ell, grad = obj.value_and_grad(theta, mode=mode)
     torch.cuda.synchronize()
     return (time.perf_counter() - t0) / steps, ell, grad
 
@@ -844,11 +844,13 @@ def main():
         rb, ib, nsteps = build_batch_runner(args.batch_seqs, 256, 8192, 0, torch.float32, cubature)
         schedb = rb.autotune(ib) if args.autotune else None
         dtb, _, repb = time_runner(rb, ib, 5, max(1, args.steps // 2), 1, bool(args.graph))
+        rb.prob.fallback_counter.zero_()
         _, _, repb = time_runner(rb, ib, 5, 1, 0, False, kernel_timing=True)
+        fb_batch = rb.fallback_count() / 5.0
         batch = {"workload": f"{args.batch_seqs} bearings-only sequences, lengths U[256, 8192], FP32 sqrt "
                              "cubature IPLS, 5 iterations, one launch sequence per pass",
                  "steps": nsteps, "ms_per_step": dtb * 1e3, "value": nsteps * 5 / dtb,
-                 "schedule": _sched(schedb, rb),
+                 "schedule": _sched(schedb, rb), "fp64_fallbacks_per_pass": fb_batch,
                  "unit": "step-it/s", "kernel_ms": {k: v[0] / v[1] for k, v in repb.items()}}
         del rb, ib
         torch.cuda.empty_cache()
@@ -859,34 +861,43 @@ def main():
         for name, dt_ in (("fp32", torch.float32), ("fp64", torch.float64)):
             log(f"gradient leg {name}: {args.grad_seqs} x T={args.grad_T}")
             obj, theta = build_grad_objective(args.grad_seqs, args.grad_T, args.grad_iter, dt_)
-            obj.prob.fallback_counter.zero_()
-            obj.tprob.fallback_counter.zero_()
-            tg, ell_g, g_g = time_gradient(obj, theta, 1, 1 if name == "fp32" else 0)
-            fbk = int(obj.prob.fallback_counter.item()) + int(obj.tprob.fallback_counter.item())
-            res = obj.fixed_point_residual(theta) if name == "fp64" else None
-            legs[name] = (tg, ell_g.double().cpu().numpy(), g_g.double().cpu().numpy(), res, fbk)
+            for mode in ("reverse", "forward"):
+                obj.prob.fallback_counter.zero_()
+                obj.tprob.fallback_counter.zero_()
+                tg, ell_g, g_g = time_gradient(obj, theta, 1, 1 if name == "fp32" else 0, mode)
+                fbk = int(obj.prob.fallback_counter.item()) + int(obj.tprob.fallback_counter.item())
+                legs[name, mode] = (tg, ell_g.double().cpu().numpy(), g_g.double().cpu().numpy(), fbk)
+            if name == "fp64":
+                conv = obj.fixed_point_residual(theta) < 1e-6
             del obj
             torch.cuda.empty_cache()
-        conv = legs["fp64"][3] < 1e-6
-        g32, g64 = legs["fp32"][2][conv].sum(0), legs["fp64"][2][conv].sum(0)
-        e32, e64 = legs["fp32"][1][conv].sum(), legs["fp64"][1][conv].sum()
         nst = args.grad_seqs * args.grad_T
+        ref = legs["fp64", "reverse"][2][conv].sum(0)
+        modes = {}
+        for mode, how in (("reverse", "N + 3 adjoint passes (FP64 adjoint arithmetic; FP32 objectives on "
+                                      "an FP64 twin of the FP32-rounded problem)"),
+                          ("forward", "P (N + 3) tangent passes in the objective's precision")):
+            t32, t64 = legs["fp32", mode][0], legs["fp64", mode][0]
+            g32, g64 = legs["fp32", mode][2][conv].sum(0), legs["fp64", mode][2][conv].sum(0)
+            e32, e64 = legs["fp32", mode][1][conv].sum(), legs["fp64", mode][1][conv].sum()
+            modes[mode] = {"how": how,
+                           "fp32_grad_evals_per_s": args.grad_seqs / t32,
+                           "fp64_grad_evals_per_s": args.grad_seqs / t64,
+                           "fp32_ms": t32 * 1e3, "fp64_ms": t64 * 1e3, "fp32_over_fp64": t64 / t32,
+                           "fp32_grad_rel_err_vs_fp64": float(np.abs(g32 - g64).max() / np.abs(g64).max()),
+                           "fp64_grad_rel_err_vs_fp64_reverse": float(np.abs(g64 - ref).max() / np.abs(ref).max()),
+                           "fp32_ell_rel_err_vs_fp64": float(abs(e32 - e64) / abs(e64)),
+                           "fp32_fp64_fallbacks": legs["fp32", mode][3]}
         grad = {"workload": f"{args.grad_seqs} bearings-only sequences x T={args.grad_T}, cubature IPLS, "
                             f"n_iter={args.grad_iter} from the simulated trajectories, gradient of the summed "
-                            "pseudo-log-likelihood in 7 parameters (implicit fixed-point rule, "
-                            "forward-mode tangent passes)",
-                "fp32_grad_evals_per_s": args.grad_seqs / legs["fp32"][0],
-                "fp64_grad_evals_per_s": args.grad_seqs / legs["fp64"][0],
-                "fp32_ms": legs["fp32"][0] * 1e3, "fp64_ms": legs["fp64"][0] * 1e3,
-                "fp32_over_fp64": legs["fp64"][0] / legs["fp32"][0],
-                "fp32_smoothed_step_its_per_s": nst * args.grad_iter / legs["fp32"][0],
+                            "pseudo-log-likelihood in 7 parameters (implicit fixed-point rule)",
+                "reverse": modes["reverse"], "forward": modes["forward"],
+                "reverse_over_forward_fp64": legs["fp64", "forward"][0] / legs["fp64", "reverse"][0],
+                "fp32_smoothed_step_its_per_s": nst * args.grad_iter / legs["fp32", "reverse"][0],
                 "fp64_sequences_at_fixed_point": int(conv.sum()),
                 "fixed_point_criterion": "max |f(x*) - x*| over the state means < 1e-6 after n_iter iterations",
-                "fp32_grad_rel_err_vs_fp64": float(np.abs(g32 - g64).max() / np.abs(g64).max()),
-                "fp32_ell_rel_err_vs_fp64": float(abs(e32 - e64) / abs(e64)),
-                "fp32_fp64_fallbacks": legs["fp32"][4],
                 "errors_over": "the sequences at a fixed point (summed gradient, summed ell)",
-                "grad_fp64": g64.tolist()}
+                "grad_fp64": ref.tolist()}
 
     north = None
     if args.north_star_seqs > 0:

@@ -1101,9 +1101,38 @@ PSM_DEV void filter_element(const Step<R, NX, NY>& s, const R (&m)[NX], const R 
 // ============================================================================
 // FP32 square-root form: FP64 re-triangularisation fallback
 // ============================================================================
+// Two ways for an FP32 square-root kernel to meet an ill-conditioned result.
+// Fallback recomputes the operation in FP64 in place (the FP64 code is part of
+// the kernel and sets its register budget).  FallbackFlag keeps the FP32
+// result and raises its workgroup's flag word; the launch sequence then runs
+// the same kernel on the same grid again with a Fallback whose guard is that
+// array of words: a workgroup whose flag is down returns at once, one whose
+// flag is up clears it and recomputes everything it wrote in the first launch
+// (every kernel reads only buffers it does not write, and workgroups write
+// disjoint outputs, so the second run replaces the first run's outputs where
+// it repeats them).  The FP32 kernels that run every pass thus carry no FP64
+// code, and the FP64 work is proportional to the workgroups that need it.
+// skip() must be reached by every thread of the workgroup (it synchronises).
 struct Fallback {
-  float thr;                    // min|diag| / max|diag| threshold (0: off)
-  unsigned long long* cnt;      // incremented per FP64 recomputation (may be null)
+  float thr;                          // min|diag| / max|diag| threshold (0: off)
+  unsigned long long* cnt;            // incremented per FP64 recomputation (may be null)
+  unsigned* guard = nullptr;          // non-null: per-workgroup flags of the first launch
+  static constexpr bool kRedo = true;
+  PSM_DEV bool skip() const {
+    if (guard == nullptr) return false;
+    const unsigned f = guard[blockIdx.x];
+    __syncthreads();
+    if (f != 0u && threadIdx.x == 0) guard[blockIdx.x] = 0u;
+    return f == 0u;
+  }
+};
+
+struct FallbackFlag {
+  float thr;
+  unsigned* flag;                     // flag[blockIdx.x] <- 1 on an ill-conditioned result
+  static constexpr bool kRedo = false;
+  PSM_DEV bool skip() const { return false; }
+  PSM_DEV void raise() const { flag[blockIdx.x] = 1u; }
 };
 
 template <typename R, bool SQRT>
@@ -1167,10 +1196,14 @@ PSM_DEV void cvt_agg(const SAgg<double, NX>& a, SAgg<float, NX>& b) {
   cvt(a.g, b.g); cvt(a.E, b.E); cvt(a.D, b.D); b.flag = a.flag;
 }
 
-template <typename R, int NX, int NY, bool SQRT>
+template <typename R, int NX, int NY, bool SQRT, typename FB>
 PSM_DEV typename AccOf<R>::type kf_step_fb(const Step<R, NX, NY>& s, const R (&y)[NY], R (&m)[NX], R (&L)[NX][NX],
-                          const Fallback& fb) {
-  if constexpr (kFallback<R, SQRT>) {
+                          const FB& fb) {
+  if constexpr (kFallback<R, SQRT> && !FB::kRedo) {
+    const double e = sqrt_kf_step<float, NX, NY>(s, y, m, L);
+    if (fb.thr > 0.f && ill_conditioned(L, fb.thr)) fb.raise();
+    return e;
+  } else if constexpr (kFallback<R, SQRT>) {
     float m0[NX], L0[NX][NX];
     copy(m, m0);
     copy(L, L0);
@@ -1191,9 +1224,12 @@ PSM_DEV typename AccOf<R>::type kf_step_fb(const Step<R, NX, NY>& s, const R (&y
   }
 }
 
-template <typename R, int NX, int NY, bool SQRT>
-PSM_DEV void absorb_fb(FAgg<R, NX>& a, const Step<R, NX, NY>& s, const R (&y)[NY], const Fallback& fb) {
-  if constexpr (kFallback<R, SQRT>) {
+template <typename R, int NX, int NY, bool SQRT, typename FB>
+PSM_DEV void absorb_fb(FAgg<R, NX>& a, const Step<R, NX, NY>& s, const R (&y)[NY], const FB& fb) {
+  if constexpr (kFallback<R, SQRT> && !FB::kRedo) {
+    absorb<float, NX, NY, true>(a, s, y);
+    if (fb.thr > 0.f && ill_conditioned(a.U, fb.thr)) fb.raise();
+  } else if constexpr (kFallback<R, SQRT>) {
     FAgg<float, NX> a0;
     copy_agg(a, a0);
     absorb<float, NX, NY, true>(a, s, y);
@@ -1213,12 +1249,14 @@ PSM_DEV void absorb_fb(FAgg<R, NX>& a, const Step<R, NX, NY>& s, const R (&y)[NY
   }
 }
 
-template <typename R, int NX, bool SQRT>
+template <typename R, int NX, bool SQRT, typename FB>
 PSM_DEV void smoother_element_fb(const R (&F)[NX][NX], const R (&cq)[NX][NX], const R (&bt)[NX],
                                  const R (&m)[NX], const R (&L)[NX][NX], SAgg<R, NX>& out,
-                                 const Fallback& fb) {
+                                 const FB& fb) {
   smoother_element<R, NX, SQRT>(F, cq, bt, m, L, out);
-  if constexpr (kFallback<R, SQRT>) {
+  if constexpr (kFallback<R, SQRT> && !FB::kRedo) {
+    if (fb.thr > 0.f && ill_conditioned(out.D, fb.thr)) fb.raise();
+  } else if constexpr (kFallback<R, SQRT>) {
     if (fb.thr > 0.f && ill_conditioned(out.D, fb.thr)) {
       double Fd[NX][NX], cqd[NX][NX], btd[NX], md[NX], Ld[NX][NX];
       cvt(F, Fd); cvt(cq, cqd); cvt(bt, btd); cvt(m, md); cvt(L, Ld);
@@ -1230,9 +1268,12 @@ PSM_DEV void smoother_element_fb(const R (&F)[NX][NX], const R (&cq)[NX][NX], co
   }
 }
 
-template <typename R, int NX, bool SQRT>
-PSM_DEV void smoother_apply_fb(const SAgg<R, NX>& e, R (&m)[NX], R (&Ds)[NX][NX], const Fallback& fb) {
-  if constexpr (kFallback<R, SQRT>) {
+template <typename R, int NX, bool SQRT, typename FB>
+PSM_DEV void smoother_apply_fb(const SAgg<R, NX>& e, R (&m)[NX], R (&Ds)[NX][NX], const FB& fb) {
+  if constexpr (kFallback<R, SQRT> && !FB::kRedo) {
+    sqrt_smoother_apply<float, NX>(e, m, Ds);
+    if (fb.thr > 0.f && ill_conditioned(Ds, fb.thr)) fb.raise();
+  } else if constexpr (kFallback<R, SQRT>) {
     float m0[NX], D0[NX][NX];
     copy(m, m0);
     copy(Ds, D0);
@@ -1253,9 +1294,13 @@ PSM_DEV void smoother_apply_fb(const SAgg<R, NX>& e, R (&m)[NX], R (&Ds)[NX][NX]
 }
 
 // e2 <- e1 (+) e2 for smoother aggregates held in registers, with the fallback
-template <typename R, int NX, bool SQRT>
-PSM_DEV void combine_into_fb(const SAgg<R, NX>& e1, SAgg<R, NX>& e2, const Fallback& fb) {
-  if constexpr (kFallback<R, SQRT>) {
+template <typename R, int NX, bool SQRT, typename FB>
+PSM_DEV void combine_into_fb(const SAgg<R, NX>& e1, SAgg<R, NX>& e2, const FB& fb) {
+  if constexpr (kFallback<R, SQRT> && !FB::kRedo) {
+    const bool plain = e2.flag == FLAG_HEAD || e1.flag == FLAG_IDENT || e2.flag == FLAG_IDENT;
+    combine_into<R, NX, SQRT>(e1, e2);
+    if (!plain && fb.thr > 0.f && ill_conditioned(e2.D, fb.thr)) fb.raise();
+  } else if constexpr (kFallback<R, SQRT>) {
     if (e2.flag == FLAG_HEAD || e1.flag == FLAG_IDENT || e2.flag == FLAG_IDENT) {
       combine_into<R, NX, SQRT>(e1, e2);
       return;

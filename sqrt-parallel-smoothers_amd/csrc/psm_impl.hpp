@@ -60,8 +60,13 @@ struct Plan {
   int64_t nchunks = 0;
   int nlevels = 0;
   ScanLevel lv[16];
-  size_t off_lin = 0, off_ell = 0, off_selem = 0, total = 0;
+  size_t off_lin = 0, off_ell = 0, off_selem = 0, off_guard = 0, total = 0;
 };
+
+// Per-workgroup flag words of the guarded second launches of the FP32
+// square-root kernels (FallbackFlag / Fallback::guard): one per workgroup of
+// the largest grid of a pass (the lane-per-state linearization).
+inline int64_t guard_words(int64_t Mp) { return (Mp + kLaneWG - 1) / kLaneWG + 1; }
 
 // Integer environment setting within [lo, hi]; unset, empty, malformed or
 // out-of-range values give dflt.
@@ -138,6 +143,8 @@ inline Plan make_plan(const psm_problem_t* p, size_t rsize, int nlin, int nqf, i
     n = s.nwg;
   }
   pl.nlevels = l;
+  pl.off_guard = off;
+  off = align_up(off + (size_t)guard_words(pl.Mp) * sizeof(unsigned));
   pl.total = off;
   return pl;
 }
@@ -343,13 +350,49 @@ struct Impl {
     return LevelMap{s.n, s.stride, s.K, s.wg, rev};
   }
 
+  // FP32 square-root passes launch each kernel twice: without FP64 code and
+  // with a FallbackFlag, then with a Fallback guarded by that flag, which
+  // redoes the kernel's work only if the first launch met an ill-conditioned
+  // result (see Fallback).  Other forms launch once with the Fallback as given.
+  // Every guarded pair leaves the flags it used down again (the second launch
+  // clears the flags it finds up), so one array serves all pairs of a pass;
+  // it is cleared once at the start of the pass.
+  struct Guards {
+    unsigned* base;
+    int64_t words;
+  };
+  static Guards guards(const Plan& pl, char* ws, hipStream_t st) {
+    Guards g{(unsigned*)(ws + pl.off_guard), guard_words(pl.Mp)};
+    if constexpr (kFallback<R, SQRT>)
+      hipLaunchKernelGGL((k_fill<unsigned>), dim3(grid_for(g.words, 256)), dim3(256), 0, st, g.base,
+                         (int)g.words, 0u);
+    return g;
+  }
+  template <typename L>
+  static void launch_fb(const Fallback& fb, Guards& g, unsigned grid, L&& launch) {
+    if constexpr (kFallback<R, SQRT>) {
+      if ((int64_t)grid > g.words) {   // no flag word per workgroup: one launch with the FP64 code
+        launch(fb);
+        return;
+      }
+      launch(FallbackFlag{fb.thr, g.base});
+      if (fb.thr > 0.f) {
+        Fallback redo = fb;
+        redo.guard = g.base;
+        launch(redo);
+      }
+    } else {
+      launch(fb);
+    }
+  }
+
   // Exclusive scan of the level-0 items (chunk aggregates, already in scan
   // order in the interleaved layout).  The level-0 down-sweep is left to the
   // apply kernel (level0_prefix): *final_off receives the level-0 prefixes
   // within each workgroup and *upper_off the complete level-1 prefixes
   // (SIZE_MAX with a single level, whose prefixes are already complete).
   template <typename Agg>
-  static int run_scan(const Plan& pl, char* ws, const Fallback& fb, hipStream_t st,
+  static int run_scan(const Plan& pl, char* ws, const Fallback& fb, Guards& g, hipStream_t st,
                       size_t* final_off, size_t* upper_off) {
     constexpr bool filt = std::is_same<Agg, FA>::value;
     for (int l = 0; l < pl.nlevels; ++l) {
@@ -361,25 +404,28 @@ struct Impl {
       const LevelMap tm = top ? LevelMap{0, 0, 1, WG, 0} : level_map(pl, l + 1, 0);
       TimedScope ts_(l == 0 ? (filt ? "k_fscan_up_l0" : "k_sscan_up_l0")
                             : (filt ? "k_fscan_up" : "k_sscan_up"), st);
-      bool launched = false;
-      if constexpr (TEAM) {
-        if (l > 0 && pl.team) {
-          hipLaunchKernelGGL((k_scan_up_team<Agg, R>), dim3((unsigned)s.nwg),
-                             dim3(TeamCfg<Agg, R>::WGT), 0, st, (const R*)in, level_map(pl, l, 0),
-                             out, tot, tm, fb);
-          launched = true;
+      launch_fb(fb, g, (unsigned)s.nwg, [&](auto f) {
+        using F = decltype(f);
+        bool launched = false;
+        if constexpr (TEAM) {
+          if (l > 0 && pl.team) {
+            hipLaunchKernelGGL((k_scan_up_team<Agg, R, F>), dim3((unsigned)s.nwg),
+                               dim3(TeamCfg<Agg, R>::WGT), 0, st, (const R*)in, level_map(pl, l, 0),
+                               out, tot, tm, f);
+            launched = true;
+          }
         }
-      }
-      if constexpr (TEAM) {
-        if (!launched && s.wg == WG / 2) {  // half-wave level 0
-          hipLaunchKernelGGL((k_scan_up<Agg, R, SQRT, WG / 2>), dim3((unsigned)s.nwg), dim3(WG / 2),
-                             0, st, (const R*)in, level_map(pl, l, 0), out, tot, tm, fb);
-          launched = true;
+        if constexpr (TEAM) {
+          if (!launched && s.wg == WG / 2) {  // half-wave level 0
+            hipLaunchKernelGGL((k_scan_up<Agg, R, SQRT, WG / 2, F>), dim3((unsigned)s.nwg), dim3(WG / 2),
+                               0, st, (const R*)in, level_map(pl, l, 0), out, tot, tm, f);
+            launched = true;
+          }
         }
-      }
-      if (!launched)
-        hipLaunchKernelGGL((k_scan_up<Agg, R, SQRT, WG>), dim3((unsigned)s.nwg), dim3(WG), 0,
-                           st, (const R*)in, level_map(pl, l, 0), out, tot, tm, fb);
+        if (!launched)
+          hipLaunchKernelGGL((k_scan_up<Agg, R, SQRT, WG, F>), dim3((unsigned)s.nwg), dim3(WG), 0,
+                             st, (const R*)in, level_map(pl, l, 0), out, tot, tm, f);
+      });
       int rc = hip_check("k_scan_up");
       if (rc) return rc;
     }
@@ -389,22 +435,25 @@ struct Impl {
       const size_t upper_off =
           (l + 1 == pl.nlevels - 1) ? pl.lv[l + 1].off_out : pl.lv[l + 1].off_in;
       TimedScope ts_(filt ? "k_fscan_down" : "k_sscan_down", st);
-      bool launched = false;
-      if constexpr (TEAM) {
-        if (l > 0 && pl.team) {
-          hipLaunchKernelGGL((k_scan_down_team<Agg, R>), dim3((unsigned)(s.nwg * s.K)),
-                             dim3(TeamCfg<Agg, R>::WGT), 0, st, (const R*)(ws + s.off_out),
-                             level_map(pl, l, 0), (const R*)(ws + upper_off),
-                             level_map(pl, l + 1, 0), (R*)(ws + s.off_in), fb);
-          launched = true;
+      launch_fb(fb, g, std::max((unsigned)(s.nwg * s.K), grid_for(s.nwg * s.K * s.wg, WG)), [&](auto f) {
+        using F = decltype(f);
+        bool launched = false;
+        if constexpr (TEAM) {
+          if (l > 0 && pl.team) {
+            hipLaunchKernelGGL((k_scan_down_team<Agg, R, F>), dim3((unsigned)(s.nwg * s.K)),
+                               dim3(TeamCfg<Agg, R>::WGT), 0, st, (const R*)(ws + s.off_out),
+                               level_map(pl, l, 0), (const R*)(ws + upper_off),
+                               level_map(pl, l + 1, 0), (R*)(ws + s.off_in), f);
+            launched = true;
+          }
         }
-      }
-      if (!launched)
-        hipLaunchKernelGGL((k_scan_down<Agg, R, SQRT, WG>),
-                           dim3((unsigned)grid_for(s.nwg * s.K * s.wg, WG)), dim3(WG), 0,
-                           st, (const R*)(ws + s.off_out), level_map(pl, l, 0),
-                           (const R*)(ws + upper_off), level_map(pl, l + 1, 0),
-                           (R*)(ws + s.off_in), fb);
+        if (!launched)
+          hipLaunchKernelGGL((k_scan_down<Agg, R, SQRT, WG, F>),
+                             dim3((unsigned)grid_for(s.nwg * s.K * s.wg, WG)), dim3(WG), 0,
+                             st, (const R*)(ws + s.off_out), level_map(pl, l, 0),
+                             (const R*)(ws + upper_off), level_map(pl, l + 1, 0),
+                             (R*)(ws + s.off_in), f);
+      });
       int rc = hip_check("k_scan_down");
       if (rc) return rc;
     }
@@ -416,7 +465,7 @@ struct Impl {
 
   static int linearize_states(const Plan& pl, const psm_model_t* tr, const psm_model_t* ob,
                               const psm_sigma_t* sg, const void* nm, const void* nL, char* ws,
-                              hipStream_t st, const Fallback& fb) {
+                              hipStream_t st, const Fallback& fb, Guards& g) {
     int use = 0;
     SigmaView<R> sv = sigma_view<R>(sg, &use);
     ModelView<R> trv = model_view<R>(tr);
@@ -424,10 +473,11 @@ struct Impl {
     trv.npts = use ? sv.npts : 0;
     obv.npts = use ? sv.npts : 0;
     TimedScope ts_("k_linearize_states", st);
-    hipLaunchKernelGGL((k_linearize_states<R, NX, NY, SQRT>), dim3(grid_for(pl.Mp, kLaneWG)),
-                       dim3(kLaneWG), 0, st, pl.M, pl.Mp, pl.chunk, trv, obv,
-                       ob ? 1 : 0, sv, use,
-                       (const R*)nm, (const R*)nL, (R*)(ws + pl.off_lin), fb);
+    launch_fb(fb, g, grid_for(pl.Mp, kLaneWG), [&](auto f) {
+      hipLaunchKernelGGL((k_linearize_states<R, NX, NY, SQRT, decltype(f)>), dim3(grid_for(pl.Mp, kLaneWG)),
+                         dim3(kLaneWG), 0, st, pl.M, pl.Mp, pl.chunk, trv, obv, ob ? 1 : 0, sv, use,
+                         (const R*)nm, (const R*)nL, (R*)(ws + pl.off_lin), f);
+    });
     return hip_check("k_linearize_states");
   }
 
@@ -444,31 +494,36 @@ struct Impl {
     char* ws = (char*)wsv;
     int rc;
     const Fallback fb{p->fallback_ratio, p->fallback_count};
+    Guards g = guards(pl, ws, st);
     if (!lin_done) {
-      rc = linearize_states(pl, tr, ob, sg, nm, nL, ws, st, fb);
+      rc = linearize_states(pl, tr, ob, sg, nm, nL, ws, st, fb, g);
       if (rc) return rc;
     }
     const R* lin = (const R*)(ws + pl.off_lin);
     {
     TimedScope ts_("k_filter_reduce", st);
-    hipLaunchKernelGGL((k_filter_reduce<R, NX, NY, SQRT>), dim3(grid_for(pl.nchunks, kLaneWG)),
-                       dim3(kLaneWG), 0, st, pl.M, pl.Mp, pl.chunk, pl.nchunks, p->state_offsets,
-                       p->n_seq, lin, (const R*)ys, (const R*)m0, (const R*)L0,
-                       (R*)(ws + pl.lv[0].off_in), level_map(pl, 0, 0), fb);
+    launch_fb(fb, g, grid_for(pl.nchunks, kLaneWG), [&](auto f) {
+      hipLaunchKernelGGL((k_filter_reduce<R, NX, NY, SQRT, decltype(f)>), dim3(grid_for(pl.nchunks, kLaneWG)),
+                         dim3(kLaneWG), 0, st, pl.M, pl.Mp, pl.chunk, pl.nchunks, p->state_offsets,
+                         p->n_seq, lin, (const R*)ys, (const R*)m0, (const R*)L0,
+                         (R*)(ws + pl.lv[0].off_in), level_map(pl, 0, 0), f);
+    });
     }
     if ((rc = hip_check("k_filter_reduce"))) return rc;
     size_t pre_off = 0, up_off = 0;
-    if ((rc = run_scan<FA>(pl, ws, fb, st, &pre_off, &up_off))) return rc;
+    if ((rc = run_scan<FA>(pl, ws, fb, g, st, &pre_off, &up_off))) return rc;
     Acc* ell_state = ell ? (Acc*)(ws + pl.off_ell) : nullptr;
     {
     TimedScope ts_("k_filter_apply", st);
-    hipLaunchKernelGGL((k_filter_apply<R, NX, NY, SQRT>), dim3(grid_for(pl.nchunks, kLaneWG)),
-                       dim3(kLaneWG), 0, st, pl.M, pl.Mp, pl.chunk, pl.nchunks, p->state_offsets,
-                       p->n_seq, lin, (const R*)ys, (const R*)m0, (const R*)L0,
-                       (const R*)(ws + pre_off), level_map(pl, 0, 0),
-                       up_off == SIZE_MAX ? nullptr : (const R*)(ws + up_off),
-                       pl.nlevels > 1 ? level_map(pl, 1, 0) : level_map(pl, 0, 0),
-                       (R*)(ws + pl.lv[0].off_in), (R*)om, (R*)oL, ell_state, fb);
+    launch_fb(fb, g, grid_for(pl.nchunks, kLaneWG), [&](auto f) {
+      hipLaunchKernelGGL((k_filter_apply<R, NX, NY, SQRT, decltype(f)>), dim3(grid_for(pl.nchunks, kLaneWG)),
+                         dim3(kLaneWG), 0, st, pl.M, pl.Mp, pl.chunk, pl.nchunks, p->state_offsets,
+                         p->n_seq, lin, (const R*)ys, (const R*)m0, (const R*)L0,
+                         (const R*)(ws + pre_off), level_map(pl, 0, 0),
+                         up_off == SIZE_MAX ? nullptr : (const R*)(ws + up_off),
+                         pl.nlevels > 1 ? level_map(pl, 1, 0) : level_map(pl, 0, 0),
+                         (R*)(ws + pl.lv[0].off_in), (R*)om, (R*)oL, ell_state, f);
+    });
     }
     if ((rc = hip_check("k_filter_apply"))) return rc;
     if (ell) {
@@ -491,29 +546,34 @@ struct Impl {
     char* ws = (char*)wsv;
     int rc;
     const Fallback fb{p->fallback_ratio, p->fallback_count};
+    Guards g = guards(pl, ws, st);
     if (!lin_done) {
-      rc = linearize_states(pl, tr, nullptr, sg, nm, nL, ws, st, fb);
+      rc = linearize_states(pl, tr, nullptr, sg, nm, nL, ws, st, fb, g);
       if (rc) return rc;
     }
     {
     TimedScope ts_("k_smoother_reduce", st);
-    hipLaunchKernelGGL((k_smoother_reduce<R, NX, NY, SQRT>), dim3(grid_for(pl.nchunks, kLaneWG)),
-                       dim3(kLaneWG), 0, st, pl.M, pl.Mp, pl.chunk, pl.nchunks, p->state_offsets,
-                       p->n_seq, (const R*)(ws + pl.off_lin), (const R*)fm, (const R*)fL,
-                       (R*)(ws + pl.off_selem), (R*)(ws + pl.lv[0].off_in), level_map(pl, 0, 1),
-                       fb);
+    launch_fb(fb, g, grid_for(pl.nchunks, kLaneWG), [&](auto f) {
+      hipLaunchKernelGGL((k_smoother_reduce<R, NX, NY, SQRT, decltype(f)>),
+                         dim3(grid_for(pl.nchunks, kLaneWG)), dim3(kLaneWG), 0, st, pl.M, pl.Mp, pl.chunk,
+                         pl.nchunks, p->state_offsets, p->n_seq, (const R*)(ws + pl.off_lin), (const R*)fm,
+                         (const R*)fL, (R*)(ws + pl.off_selem), (R*)(ws + pl.lv[0].off_in),
+                         level_map(pl, 0, 1), f);
+    });
     }
     if ((rc = hip_check("k_smoother_reduce"))) return rc;
     size_t pre_off = 0, up_off = 0;
-    if ((rc = run_scan<SA>(pl, ws, fb, st, &pre_off, &up_off))) return rc;
+    if ((rc = run_scan<SA>(pl, ws, fb, g, st, &pre_off, &up_off))) return rc;
     {
       TimedScope ts_("k_smoother_apply", st);
-      hipLaunchKernelGGL((k_smoother_apply<R, NX, SQRT>), dim3(grid_for(pl.nchunks, kLaneWG)),
-                         dim3(kLaneWG), 0, st, pl.M, pl.Mp, pl.chunk, pl.nchunks,
-                         (const R*)(ws + pl.off_selem), (const R*)(ws + pre_off),
-                         level_map(pl, 0, 1), up_off == SIZE_MAX ? nullptr : (const R*)(ws + up_off),
-                         pl.nlevels > 1 ? level_map(pl, 1, 0) : level_map(pl, 0, 0),
-                         (R*)(ws + pl.lv[0].off_in), (R*)om, (R*)oL, fb);
+      launch_fb(fb, g, grid_for(pl.nchunks, kLaneWG), [&](auto f) {
+        hipLaunchKernelGGL((k_smoother_apply<R, NX, SQRT, decltype(f)>), dim3(grid_for(pl.nchunks, kLaneWG)),
+                           dim3(kLaneWG), 0, st, pl.M, pl.Mp, pl.chunk, pl.nchunks,
+                           (const R*)(ws + pl.off_selem), (const R*)(ws + pre_off),
+                           level_map(pl, 0, 1), up_off == SIZE_MAX ? nullptr : (const R*)(ws + up_off),
+                           pl.nlevels > 1 ? level_map(pl, 1, 0) : level_map(pl, 0, 0),
+                           (R*)(ws + pl.lv[0].off_in), (R*)om, (R*)oL, f);
+      });
     }
     return hip_check("k_smoother_apply");
   }
@@ -552,13 +612,16 @@ struct Impl {
     if (pl.M <= 0 || S <= 0) return PSM_OK;
     char* ws = (char*)wsv;
     const Fallback fb{p->fallback_ratio, p->fallback_count};
-    int rc = linearize_states(pl, tr, nullptr, sg, nm, nL, ws, st, fb);
+    Guards g = guards(pl, ws, st);
+    int rc = linearize_states(pl, tr, nullptr, sg, nm, nL, ws, st, fb, g);
     if (rc) return rc;
     // smoother elements at every state (the chunk aggregates it also forms are not used)
-    hipLaunchKernelGGL((k_smoother_reduce<R, NX, NY, SQRT>), dim3(grid_for(pl.nchunks, kLaneWG)),
-                       dim3(kLaneWG), 0, st, pl.M, pl.Mp, pl.chunk, pl.nchunks, p->state_offsets,
-                       p->n_seq, (const R*)(ws + pl.off_lin), (const R*)fm, (const R*)fL,
-                       (R*)(ws + pl.off_selem), (R*)(ws + pl.lv[0].off_in), level_map(pl, 0, 1), fb);
+    launch_fb(fb, g, grid_for(pl.nchunks, kLaneWG), [&](auto f) {
+      hipLaunchKernelGGL((k_smoother_reduce<R, NX, NY, SQRT, decltype(f)>), dim3(grid_for(pl.nchunks, kLaneWG)),
+                         dim3(kLaneWG), 0, st, pl.M, pl.Mp, pl.chunk, pl.nchunks, p->state_offsets,
+                         p->n_seq, (const R*)(ws + pl.off_lin), (const R*)fm, (const R*)fL,
+                         (R*)(ws + pl.off_selem), (R*)(ws + pl.lv[0].off_in), level_map(pl, 0, 1), f);
+    });
     if ((rc = hip_check("k_smoother_reduce"))) return rc;
     const int64_t lanes = sp.nch * S;
     hipLaunchKernelGGL((k_sample_reduce<R, NX, SQRT>), dim3(grid_for(lanes, kLaneWG)), dim3(kLaneWG),

@@ -129,6 +129,14 @@ struct Lin {
   int use_sigma;
 };
 
+// p[0..n) <- v (flag words and counters; a kernel rather than hipMemsetAsync,
+// so that a captured graph replays it like any other launch)
+template <typename T>
+__global__ void k_fill(T* __restrict__ p, int n, T v) {
+  const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+  if (i < n) p[i] = v;
+}
+
 // ============================================================================
 // 1. linearization at every nominal state
 // ============================================================================
@@ -192,11 +200,23 @@ PSM_DEV void linearize_point_wide(const ModelView<R>& md, const SigmaView<R>& sp
   widen(bW, b);
 }
 
-template <typename R, int NIN, int NOUT, bool SQRT>
+template <typename R, int NIN, int NOUT, bool SQRT, typename FB>
 PSM_DEV void linearize_point_w(const ModelView<R>& md, const SigmaView<R>& sp, int use_sigma,
                                const R (&m)[NIN], const R (&Lx)[NIN][NIN], R (&F)[NOUT][NIN],
-                               R (&C)[NOUT][NOUT], R (&b)[NOUT], const Fallback& fb) {
-  if constexpr (IsSingle<R>::value) {
+                               R (&C)[NOUT][NOUT], R (&b)[NOUT], const FB& fb) {
+  if constexpr (kFallback<R, SQRT> && !FB::kRedo) {
+    // FP32 square-root pass without FP64 code: a point the centred regression
+    // cannot take (conditional moments, a negative weight) or whose factor is
+    // ill-conditioned raises the flag, and the guarded second launch
+    // linearizes every point again with the FP64 path below
+    if (!use_sigma) {
+      linearize_extended<R, NIN, NOUT>(md, m, F, C, b);
+      return;
+    }
+    const bool done = md.kind != PSM_MODEL_CONDITIONAL &&
+                      linearize_sigma_centred<R, NIN, NOUT, true>(md, sp, m, Lx, F, C, b);
+    if (!done || (fb.thr > 0.0f && ill_conditioned(C, fb.thr))) fb.raise();
+  } else if constexpr (IsSingle<R>::value) {
     if (!use_sigma) {
       linearize_extended<R, NIN, NOUT>(md, m, F, C, b);
       return;
@@ -225,11 +245,12 @@ PSM_DEV void linearize_point_w(const ModelView<R>& md, const SigmaView<R>& sp, i
 
 // Thread t fills position t of the state-interleaved buffer (coalesced stores);
 // its state is the inverse of StateMap.
-template <typename R, int NX, int NY, bool SQRT>
+template <typename R, int NX, int NY, bool SQRT, typename FB = Fallback>
 __global__ __launch_bounds__(kLaneWG) void k_linearize_states(
     int64_t M, int64_t Mp, int C, ModelView<R> tr, ModelView<R> ob, int do_obs, SigmaView<R> sp,
     int use_sigma, const R* __restrict__ nom_m, const R* __restrict__ nom_L, R* __restrict__ lin,
-    Fallback fb) {
+    FB fb) {
+  if (fb.skip()) return;
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= Mp) return;
   const int64_t g = (int64_t)C << 6;
@@ -263,11 +284,12 @@ __global__ __launch_bounds__(kLaneWG) void k_linearize_states(
 // ============================================================================
 // 2. chunk aggregates of the filter
 // ============================================================================
-template <typename R, int NX, int NY, bool SQRT>
+template <typename R, int NX, int NY, bool SQRT, typename FB = Fallback>
 __global__ __launch_bounds__(kLaneWG) void k_filter_reduce(
     int64_t M, int64_t Mp, int chunk, int64_t nchunks, const int64_t* __restrict__ soff, int nseq,
     const R* __restrict__ lin, const R* __restrict__ ys, const R* __restrict__ m0,
-    const R* __restrict__ L0, R* __restrict__ agg, LevelMap lm, Fallback fb) {
+    const R* __restrict__ L0, R* __restrict__ agg, LevelMap lm, FB fb) {
+  if (fb.skip()) return;
   const StateMap sm{chunk};
   const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= nchunks) return;
@@ -308,10 +330,11 @@ __global__ __launch_bounds__(kLaneWG) void k_filter_reduce(
 // inclusive total becomes item w of the next level (tm).  Aggregates live in
 // LDS / HBM, never whole in registers: the operator reads its operands from
 // memory as it needs them.
-template <typename Agg, typename R, bool SQRT, int WG = scan_wg<Agg, R>()>
+template <typename Agg, typename R, bool SQRT, int WG = scan_wg<Agg, R>(), typename FB = Fallback>
 __global__ __launch_bounds__(WG) void k_scan_up(const R* __restrict__ in, LevelMap lm,
                                                 R* __restrict__ out_excl, R* __restrict__ totals,
-                                                LevelMap tm, Fallback fb) {
+                                                LevelMap tm, FB fb) {
+  if (fb.skip()) return;
   using V = typename ViewOf<Agg>::type;
   constexpr int NX = Agg_nx<Agg>::value;
   constexpr int NQ = Agg::NQ;
@@ -377,10 +400,11 @@ __global__ __launch_bounds__(WG) void k_scan_up(const R* __restrict__ in, LevelM
 // the next level (um).  The combinations are independent, so every thread
 // takes one storage position q of the level (consecutive threads, consecutive
 // positions) and recovers its scan position p and workgroup w from q.
-template <typename Agg, typename R, bool SQRT, int WG = scan_wg<Agg, R>()>
+template <typename Agg, typename R, bool SQRT, int WG = scan_wg<Agg, R>(), typename FB = Fallback>
 __global__ __launch_bounds__(WG) void k_scan_down(const R* __restrict__ out_excl, LevelMap lm,
                                                   const R* __restrict__ upper, LevelMap um,
-                                                  R* __restrict__ dst, Fallback fb) {
+                                                  R* __restrict__ dst, FB fb) {
+  if (fb.skip()) return;
   using V = typename ViewOf<Agg>::type;
   constexpr int NX = Agg_nx<Agg>::value;
   const int64_t q = (int64_t)blockIdx.x * WG + threadIdx.x;
@@ -408,10 +432,11 @@ struct TeamCfg {
   static constexpr int SCR = team_scratch_bytes<V>();
 };
 
-template <typename Agg, typename R>
+template <typename Agg, typename R, typename FB = Fallback>
 __global__ __launch_bounds__((TeamCfg<Agg, R>::WGT)) void k_scan_up_team(
     const R* __restrict__ in, LevelMap lm, R* __restrict__ out_excl, R* __restrict__ totals,
-    LevelMap tm, Fallback fb) {
+    LevelMap tm, FB fb) {
+  if (fb.skip()) return;
   using Cfg = TeamCfg<Agg, R>;
   using V = typename Cfg::V;
   constexpr int NX = Agg_nx<Agg>::value;
@@ -473,10 +498,11 @@ __global__ __launch_bounds__((TeamCfg<Agg, R>::WGT)) void k_scan_up_team(
 
 // One item per team: block b covers workgroup b / K of the level and item
 // b % K of each of its teams.
-template <typename Agg, typename R>
+template <typename Agg, typename R, typename FB = Fallback>
 __global__ __launch_bounds__((TeamCfg<Agg, R>::WGT)) void k_scan_down_team(
     const R* __restrict__ out_excl, LevelMap lm, const R* __restrict__ upper, LevelMap um,
-    R* __restrict__ dst, Fallback fb) {
+    R* __restrict__ dst, FB fb) {
+  if (fb.skip()) return;
   using Cfg = TeamCfg<Agg, R>;
   using V = typename Cfg::V;
   constexpr int NX = Agg_nx<Agg>::value;
@@ -509,10 +535,10 @@ __global__ __launch_bounds__((TeamCfg<Agg, R>::WGT)) void k_scan_down_team(
 // into joined (the free level-0 input buffer).  upper == nullptr: a one-level
 // scan, whose prefixes are already complete.  Returns the buffer holding the
 // prefix and its position *pc.
-template <typename Agg, typename R, bool SQRT>
+template <typename Agg, typename R, bool SQRT, typename FB>
 PSM_DEV const R* level0_prefix(int64_t c, const R* __restrict__ prefix, const LevelMap& lm,
                                const R* __restrict__ upper, const LevelMap& um,
-                               R* __restrict__ joined, const Fallback& fb, int64_t* pc) {
+                               R* __restrict__ joined, const FB& fb, int64_t* pc) {
   using V = typename ViewOf<Agg>::type;
   constexpr int NX = Agg_nx<Agg>::value;
   const int64_t p = lm.rev ? lm.n - 1 - c : c;
@@ -524,13 +550,14 @@ PSM_DEV const R* level0_prefix(int64_t c, const R* __restrict__ prefix, const Le
   return joined;
 }
 
-template <typename R, int NX, int NY, bool SQRT>
+template <typename R, int NX, int NY, bool SQRT, typename FB = Fallback>
 __global__ __launch_bounds__(kLaneWG) void k_filter_apply(
     int64_t M, int64_t Mp, int chunk, int64_t nchunks, const int64_t* __restrict__ soff, int nseq,
     const R* __restrict__ lin, const R* __restrict__ ys, const R* __restrict__ m0,
     const R* __restrict__ L0, const R* __restrict__ prefix, LevelMap lm,
     const R* __restrict__ upper, LevelMap um, R* __restrict__ joined, R* __restrict__ out_m,
-    R* __restrict__ out_L, typename AccOf<R>::type* __restrict__ ell_state, Fallback fb) {
+    R* __restrict__ out_L, typename AccOf<R>::type* __restrict__ ell_state, FB fb) {
+  if (fb.skip()) return;
   using Acc = typename AccOf<R>::type;
   const StateMap sm{chunk};
   const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -589,11 +616,12 @@ __global__ __launch_bounds__(BS) void k_ell_reduce(const Acc* __restrict__ ell_s
 // ============================================================================
 // smoother passes
 // ============================================================================
-template <typename R, int NX, int NY, bool SQRT>
+template <typename R, int NX, int NY, bool SQRT, typename FB = Fallback>
 __global__ __launch_bounds__(kLaneWG) void k_smoother_reduce(
     int64_t M, int64_t Mp, int chunk, int64_t nchunks, const int64_t* __restrict__ soff, int nseq,
     const R* __restrict__ lin, const R* __restrict__ fm, const R* __restrict__ fL,
-    R* __restrict__ elems, R* __restrict__ agg, LevelMap lm, Fallback fb) {
+    R* __restrict__ elems, R* __restrict__ agg, LevelMap lm, FB fb) {
+  if (fb.skip()) return;
   const StateMap sm{chunk};
   const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= nchunks) return;
@@ -629,11 +657,12 @@ __global__ __launch_bounds__(kLaneWG) void k_smoother_reduce(
   store_agg(agg, lm.stride, lm.pos(c), acc);
 }
 
-template <typename R, int NX, bool SQRT>
+template <typename R, int NX, bool SQRT, typename FB = Fallback>
 __global__ __launch_bounds__(kLaneWG) void k_smoother_apply(
     int64_t M, int64_t Mp, int chunk, int64_t nchunks, const R* __restrict__ elems,
     const R* __restrict__ prefix, LevelMap lm, const R* __restrict__ upper, LevelMap um,
-    R* __restrict__ joined, R* __restrict__ out_m, R* __restrict__ out_L, Fallback fb) {
+    R* __restrict__ joined, R* __restrict__ out_m, R* __restrict__ out_L, FB fb) {
+  if (fb.skip()) return;
   const StateMap sm{chunk};
   const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= nchunks) return;

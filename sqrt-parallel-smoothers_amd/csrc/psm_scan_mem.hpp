@@ -454,11 +454,14 @@ PSM_DEV bool view_ill(const SView<R, NX>& v, float thr) {
 
 // combine_mem with the FP64 re-triangularisation fallback (FP32 square-root
 // form).  out must not alias e2 here, so the operands survive for the redo.
-template <typename R, int NX, bool SQRT, typename V>
-PSM_DEV void combine_mem_fb(const V& e1, const V& e2, const V& out, const Fallback& fb) {
+template <typename R, int NX, bool SQRT, typename V, typename FB>
+PSM_DEV void combine_mem_fb(const V& e1, const V& e2, const V& out, const FB& fb) {
   const int f1 = e1.flag(), f2 = e2.flag();
   combine_mem<R, NX, SQRT>(e1, e2, out);
-  if constexpr (kFallback<R, SQRT>) {
+  if constexpr (kFallback<R, SQRT> && !FB::kRedo) {
+    if (fb.thr > 0.f && f2 != FLAG_HEAD && f1 != FLAG_IDENT && f2 != FLAG_IDENT && view_ill(out, fb.thr))
+      fb.raise();
+  } else if constexpr (kFallback<R, SQRT>) {
     if (fb.thr > 0.f && f2 != FLAG_HEAD && f1 != FLAG_IDENT && f2 != FLAG_IDENT &&
         view_ill(out, fb.thr)) {
       using AD = typename std::conditional<std::is_same<V, FView<R, NX>>::value, FAgg<double, NX>,

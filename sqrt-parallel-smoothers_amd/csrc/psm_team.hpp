@@ -337,9 +337,9 @@ template <typename V> constexpr int team_scratch_bytes() {
 // One combination by a team: the flag cases of combine_mem, the cooperative
 // operator, and for FP32 the recomputation in FP64 when the combined factor's
 // min|diag| / max|diag| falls below the threshold (or is not finite).
-template <typename R, int NX, typename V>
+template <typename R, int NX, typename V, typename FB>
 PSM_DEV void team_combine(const V& e1, const V& e2, const V& out, int r, void* scratch,
-                          const Fallback& fb) {
+                          const FB& fb) {
   team_sync();
   const int f1 = e1.flag(), f2 = e2.flag();
   if (f2 == FLAG_HEAD || f1 == FLAG_IDENT) {
@@ -356,7 +356,9 @@ PSM_DEV void team_combine(const V& e1, const V& e2, const V& out, int r, void* s
     }
     team_sync();
     const R ratio = team_combine_core<R, R, NX>(s1, s2, out, r);
-    if constexpr (kFallback<R, true>) {
+    if constexpr (kFallback<R, true> && !FB::kRedo) {
+      if (fb.thr > 0.f && !(ratio >= R(fb.thr)) && r == 0) fb.raise();
+    } else if constexpr (kFallback<R, true>) {
       if (fb.thr > 0.f && !(ratio >= R(fb.thr))) {
         team_combine_core<R, double, NX>(s1, s2, out, r);
         if (r == 0) count_fallback(fb);
