@@ -1284,14 +1284,19 @@ __global__ __launch_bounds__(64) void k_adj_chain(int64_t M, int64_t Mp, const i
 // ---------------------------------------------------------------- through the linearizations
 // Nominal cotangents out_m / out_L (FP64 AoS; plus add_m / add_L when given)
 // and per-state model-parameter cotangents th.
-template <typename R, int NX, int NY>
+// Two launches per VJP, one per model (PART 0: the transition of step i + 1,
+// linearized at nominal i; PART 1: the observation at state i), each with the
+// extended (US = 0) or the sigma-point (US = 1) path compiled alone: one
+// thread holding both models and both paths needed 512 registers and spilled
+// 1.7 KB per lane.  PART 0 writes the nominal's cotangents (plus add_m /
+// add_L), PART 1 adds its own.
+template <typename R, int NX, int NY, int US, int PART>
 __global__ __launch_bounds__(64) void k_adj_lin(int64_t M, int64_t Mp, const int64_t* __restrict__ soff,
                                                 int nseq, ModelView<D, R> tr, ModelView<D, R> ob, SigmaView<D> sp,
-                                                int use_sigma, const R* __restrict__ nom_m,
-                                                const R* __restrict__ nom_L, const D* __restrict__ lc,
-                                                const D* __restrict__ add_m, const D* __restrict__ add_L,
-                                                D* __restrict__ out_m, D* __restrict__ out_L, D* __restrict__ th,
-                                                int64_t tst) {
+                                                const R* __restrict__ nom_m, const R* __restrict__ nom_L,
+                                                const D* __restrict__ lc, const D* __restrict__ add_m,
+                                                const D* __restrict__ add_L, D* __restrict__ out_m,
+                                                D* __restrict__ out_L, D* __restrict__ th, int64_t tst) {
   using LC = Lco<NX, NY>;
   using TH = Th<NX, NY>;
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -1303,9 +1308,9 @@ __global__ __launch_bounds__(64) void k_adj_lin(int64_t M, int64_t Mp, const int
   D m[NX], L[NX][NX], mb[NX], Lb[NX][NX];
   ld_cv(nom_m, i, m);
   ld_cv(nom_L, i, L);
-  set_zero(mb);
-  set_zero(Lb);
-  {
+  if constexpr (PART == 0) {
+    set_zero(mb);
+    set_zero(Lb);
     D pt[NX * NX], qmb[NX], qcb[NX][NX];
 #pragma unroll
     for (int a = 0; a < NX * NX; ++a) pt[a] = 0.0;
@@ -1315,18 +1320,30 @@ __global__ __launch_bounds__(64) void k_adj_lin(int64_t M, int64_t Mp, const int
       D F[NX][NX], C[NX][NX], b[NX], Fb[NX][NX], bb[NX], Cb[NX][NX];
       ModelView<D, R> mt = tr;
       mt.row = i;
-      lin_fwd<NX, NX>(mt, sp, use_sigma, m, L, F, C, b);
+      lin_fwd<NX, NX>(mt, sp, US, m, L, F, C, b);
       ldm(lc, Mp, t, LC::F, Fb);
       ldv(lc, Mp, t, LC::B, bb);
       ldm(lc, Mp, t, LC::CQ, Cb);
-      lin_vjp<NX, NX>(mt, sp, use_sigma, m, L, C, Fb, bb, Cb, mb, Lb, pt, qmb, qcb);
+      lin_vjp<NX, NX>(mt, sp, US, m, L, C, Fb, bb, Cb, mb, Lb, pt, qmb, qcb);
     }
 #pragma unroll
     for (int a = 0; a < NX * NX; ++a) th[(int64_t)(TH::TP + a) * tst + t] = pt[a];
     stv(th, tst, t, TH::TQM, qmb);
     stm(th, tst, t, TH::TQC, qcb);
-  }
-  {
+    if (add_m != nullptr) {
+      D am[NX], aL[NX][NX];
+      ld_cv(add_m, i, am);
+      ld_cv(add_L, i, aL);
+#pragma unroll
+      for (int a = 0; a < NX; ++a) {
+        mb[a] += am[a];
+#pragma unroll
+        for (int c = 0; c < NX; ++c) Lb[a][c] += aL[a][c];
+      }
+    }
+  } else {
+    ld_cv(out_m, i, mb);
+    ld_cv(out_L, i, Lb);
     D po[NY * NX], qmb[NY], qcb[NY][NY];
 #pragma unroll
     for (int a = 0; a < NY * NX; ++a) po[a] = 0.0;
@@ -1336,27 +1353,16 @@ __global__ __launch_bounds__(64) void k_adj_lin(int64_t M, int64_t Mp, const int
       D H[NY][NX], C[NY][NY], c[NY], Hb[NY][NX], cb[NY], Cb[NY][NY];
       ModelView<D, R> mo = ob;
       mo.row = i;
-      lin_fwd<NX, NY>(mo, sp, use_sigma, m, L, H, C, c);
+      lin_fwd<NX, NY>(mo, sp, US, m, L, H, C, c);
       ldm2(lc, Mp, t, LC::H, Hb);
       ldv(lc, Mp, t, LC::C, cb);
       ldm(lc, Mp, t, LC::CR, Cb);
-      lin_vjp<NX, NY>(mo, sp, use_sigma, m, L, C, Hb, cb, Cb, mb, Lb, po, qmb, qcb);
+      lin_vjp<NX, NY>(mo, sp, US, m, L, C, Hb, cb, Cb, mb, Lb, po, qmb, qcb);
     }
 #pragma unroll
     for (int a = 0; a < NY * NX; ++a) th[(int64_t)(TH::OP + a) * tst + t] = po[a];
     stv(th, tst, t, TH::OQM, qmb);
     stm(th, tst, t, TH::OQC, qcb);
-  }
-  if (add_m != nullptr) {
-    D am[NX], aL[NX][NX];
-    ld_cv(add_m, i, am);
-    ld_cv(add_L, i, aL);
-#pragma unroll
-    for (int a = 0; a < NX; ++a) {
-      mb[a] += am[a];
-#pragma unroll
-      for (int c = 0; c < NX; ++c) Lb[a][c] += aL[a][c];
-    }
   }
   st_aos(out_m, i, mb);
   st_aos(out_L, i, Lb);

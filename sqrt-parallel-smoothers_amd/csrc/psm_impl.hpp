@@ -869,9 +869,20 @@ struct Impl {
       }
       {
         TimedScope ts_("k_adj_lin", st);
-        hipLaunchKernelGGL((k_adj_lin<R, NX, NY>), dim3(g), dim3(64), 0, st, a.M, a.Mp, p->state_offsets,
-                           p->n_seq, trv, obv, sv, use, (const R*)nm, (const R*)nL, (const double*)lc, add_m,
-                           add_L, out_m, out_L, th, a.Mp);
+        auto lin = [&](auto us, auto part) {
+          hipLaunchKernelGGL((k_adj_lin<R, NX, NY, decltype(us)::value, decltype(part)::value>), dim3(g), dim3(64),
+                             0, st, a.M, a.Mp, p->state_offsets, p->n_seq, trv, obv, sv, (const R*)nm,
+                             (const R*)nL, (const double*)lc, add_m, add_L, out_m, out_L, th, a.Mp);
+        };
+        using I0 = std::integral_constant<int, 0>;
+        using I1 = std::integral_constant<int, 1>;
+        if (use) {
+          lin(I1{}, I0{});
+          lin(I1{}, I1{});
+        } else {
+          lin(I0{}, I0{});
+          lin(I0{}, I1{});
+        }
         if ((rc = hip_check("k_adj_lin"))) return rc;
       }
       if (theta != nullptr) {
