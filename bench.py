@@ -65,12 +65,14 @@ FP32_BAR = 1e-4   # smoothed-mean relative error of the FP32 path the metric req
 PMC_NAMES = {
     "k_fscan_up_l0": "psm::k_scan_up<psm::FAgg<double, 5>, double, true, {wg}>",
     "k_sscan_up_l0": "psm::k_scan_up<psm::SAgg<double, 5>, double, true, {wg}>",
+    "k_fscan_up_l0_quad": "psm::k_scan_up_quad<psm::FAgg<double, 5>, double, {wg}>",
+    "k_sscan_up_l0_quad": "psm::k_scan_up_quad<psm::SAgg<double, 5>, double, {wg}>",
     "k_filter_reduce": "psm::k_filter_reduce<double, 5, 2, true>",
     "k_filter_apply": "psm::k_filter_apply<double, 5, 2, true>",
     "k_smoother_reduce": "psm::k_smoother_reduce<double, 5, 2, true>",
     "k_smoother_apply": "psm::k_smoother_apply<double, 5, true>",
 }
-SCHED_KEYS = ("chunk", "scan_k", "team_upper_levels", "half_wave_level0")
+SCHED_KEYS = ("chunk", "scan_k", "team_upper_levels", "half_wave_level0", "quad_level0")
 
 
 def measured_traffic(label, sched):
@@ -79,6 +81,8 @@ def measured_traffic(label, sched):
     size); None when no such summary is present or when it was taken with
     another launch schedule than `sched`."""
     path = os.path.join(ROOT, "profiles", "pmc_traffic_bearings_ieks_T131072.json")
+    if sched.get("quad_level0") and label + "_quad" in PMC_NAMES:
+        label += "_quad"
     name = PMC_NAMES.get(label, "").format(wg=32 if sched.get("half_wave_level0") else 64)
     try:
         with open(path) as fh:
@@ -644,11 +648,12 @@ def north_star_smoothing(n_seq, T, n_iter, steps, warmup, seed=5):
 
 def _sched(tuned, runner):
     """The launch schedule a runner uses: chunk, scan fan-in, team upper levels,
-    half-wave level 0 (scan_k 0: the library default, fan-in 2 with team levels)."""
+    half-wave level 0, quad-cooperative level 0 (scan_k 0: the library default,
+    fan-in 2 with team levels)."""
     k = int(runner.prob.struct.scan_k)
     team = bool(k & 0x100) or (k & 0x1FF) == 0
     return {"chunk": runner.chunk_size(), "scan_k": (k & 0xFF) or (2 if team else 4), "team_upper_levels": team,
-            "half_wave_level0": bool(k & 0x200),
+            "half_wave_level0": bool(k & 0x200), "quad_level0": bool(k & 0x400),
             "autotuned": tuned is not None, "tuned_ms_per_pass": None if tuned is None else tuned[2]}
 
 
@@ -763,7 +768,7 @@ def main():
     ap.add_argument("--autotune", type=int, default=1,
                     help="pick the launch schedule with psm_autotune before timing (1) or keep the default (0)")
     ap.add_argument("--schedule", type=str, default="",
-                    help="fixed schedule 'chunk,scan_k,team[,half]' for the T=2^17 legs (implies --autotune 0)")
+                    help="fixed schedule 'chunk,scan_k,team[,half[,quad]]' for the T=2^17 legs (implies --autotune 0)")
     ap.add_argument("--sweep", type=str, default="10,12,14,16",
                     help="comma-separated log2 T of the IEKS T-sweep ('' to skip)")
     ap.add_argument("--fp32-errors", type=str, default="14,16,18",
@@ -805,7 +810,8 @@ def main():
     def apply_fixed(r):
         if fixed:
             r.prob.set_schedule(fixed[0], fixed[1] | (0x100 if fixed[2] else 0)
-                                | (0x200 if len(fixed) > 3 and fixed[3] else 0))
+                                | (0x200 if len(fixed) > 3 and fixed[3] else 0)
+                                | (0x400 if len(fixed) > 4 and fixed[4] else 0))
 
     r64, init64 = build_runner(T, ys, torch.float64)
     apply_fixed(r64)

@@ -101,6 +101,7 @@ typedef struct psm_sigma {
 #define PSM_SCAN_K_MASK 0xff
 #define PSM_SCAN_TEAM 0x100
 #define PSM_SCAN_HALF 0x200
+#define PSM_SCAN_QUAD 0x400
 #define PSM_SCAN_TOP_SHIFT 16
 #define PSM_SCAN_TOP_MASK 0xff0000
 
@@ -129,6 +130,10 @@ typedef struct psm_problem {
    * PSM_SCAN_HALF: level 0 of the scan runs 32-lane workgroups (square-root
    * form, float / double): twice the workgroups, half the LDS each, one
    * scan-over-lanes step less;
+   * PSM_SCAN_QUAD: level 0 of the scan runs quad-cooperative kernels (four
+   * lanes per combination, 64 or, with PSM_SCAN_HALF, 32 quads per
+   * workgroup; square-root form, float / double): the level layout is the
+   * same, each combination's chain of dependent operations about a third;
    * bits 16-23 (PSM_SCAN_TOP_MASK): a level whose items fit one workgroup
    * with at most this many items per lane is the top level (values below the
    * fan-in, 0 included, mean the fan-in) */
@@ -368,7 +373,8 @@ int psm_mean_sq_diff(int32_t dtype, int64_t n, const void* a, const void* b, dou
  * XLA's).  Times psm_filter_smoother (eager, on `stream`) for candidate
  * schedules whose workspace fits workspace_bytes: first chunk in {2, 4, 8, 16,
  * 32} states per lane x scan fan-in scan_k in {2, 4, 8}, with and without
- * PSM_SCAN_TEAM and PSM_SCAN_HALF (square-root form in float / double); then
+ * PSM_SCAN_TEAM and PSM_SCAN_HALF, and with PSM_SCAN_TEAM | PSM_SCAN_QUAD
+ * with and without PSM_SCAN_HALF (square-root form in float / double); then
  * the chunks at 5/8 .. 7/4 of the best chunk (those that are not powers of
  * two, at most 64) with the three best scan_k values.  Returns the fastest in
  * *best_chunk / *best_scan_k (and its time per pass in *best_ms, may be

@@ -826,7 +826,8 @@ int psm_autotune(const psm_problem_t* p, const psm_model_t* trans, const psm_mod
     return PSM_OK;
   };
   // stage 1: power-of-two chunks x fan-in {2, 4, 8} x {per-lane, team upper
-  // levels, half-wave level 0, both}; stage 2: chunks between the stage-1
+  // levels, half-wave level 0, both, team + quad level 0, team + quad +
+  // half-wave}; stage 2: chunks between the stage-1
   // sizes (5/8 .. 7/4 of the best one) with the three best scan modes.  Chunk
   // sizes that are not powers of two avoid a nearly empty last workgroup when
   // the state count is just above a power of two (T + 1 states).
@@ -835,10 +836,12 @@ int psm_autotune(const psm_problem_t* p, const psm_model_t* trans, const psm_mod
   const bool team_ok = p->form == PSM_FORM_SQRT && (p->dtype == PSM_F32 || p->dtype == PSM_F64);
   std::vector<TuneResult> res;
   for (int32_t c : chunks) {
-    for (int32_t kk = 0; kk < 12 && rc == 0; ++kk) {
-      // kk / 3: 0 per-lane, 1 team upper levels, 2 half-wave level 0, 3 both
+    static const int32_t kModes[] = {0, PSM_SCAN_TEAM, PSM_SCAN_HALF, PSM_SCAN_TEAM | PSM_SCAN_HALF,
+                                     PSM_SCAN_TEAM | PSM_SCAN_QUAD,
+                                     PSM_SCAN_TEAM | PSM_SCAN_QUAD | PSM_SCAN_HALF};
+    for (int32_t kk = 0; kk < 18 && rc == 0; ++kk) {
       const int32_t mode = kk / 3;
-      const int32_t k = ks[kk % 3] | ((mode & 1) ? PSM_SCAN_TEAM : 0) | ((mode & 2) ? PSM_SCAN_HALF : 0);
+      const int32_t k = ks[kk % 3] | kModes[mode];
       if (mode > 0 && !team_ok) continue;
       float ms;
       rc = time_one(c, k, &ms);

@@ -57,6 +57,7 @@ struct Plan {
   int chunk = 0;
   int wg = 0;      // scan workgroup size
   bool team = false;  // team-cooperative kernels above level 0
+  bool quad = false;  // quad-cooperative level-0 up-sweep
   int64_t nchunks = 0;
   int nlevels = 0;
   ScanLevel lv[16];
@@ -207,8 +208,8 @@ struct OpsTable {
                       const void*, const void*, const void*, void*, void*, hipStream_t);
   // the launch schedule a problem resolves to (environment defaults applied),
   // packed: chunk | level-0 fan-in << 16 | levels << 24 | team << 32 |
-  // level-0 workgroup size << 33; a captured launch sequence is valid only for
-  // the same signature
+  // level-0 workgroup size << 33 | quad level 0 << 49; a captured launch
+  // sequence is valid only for the same signature
   uint64_t (*plan_signature)(const psm_problem_t*);
   // reverse-mode adjoint (square-root form, float / double primal)
   size_t (*adjoint_workspace_bytes)(const psm_problem_t*);
@@ -250,6 +251,7 @@ struct Impl {
     Plan pl = make_plan(p, sizeof(R), NLIN, FA::NQ, SA::NQ, sizeof(Acc), half ? WG / 2 : WG,
                         team ? kTeamVL : WG, team ? 2 : 4);
     pl.team = team;
+    pl.quad = TEAM && (p->scan_k & PSM_SCAN_QUAD) != 0;
     return pl;
   }
 
@@ -262,6 +264,7 @@ struct Impl {
     sig |= (uint64_t)(pl.nlevels & 0xff) << 24;
     sig |= (uint64_t)(pl.team ? 1 : 0) << 32;
     sig |= (uint64_t)(pl.lv[0].wg & 0xffff) << 33;
+    sig |= (uint64_t)(pl.quad ? 1 : 0) << 49;
     for (int l = 1; l < pl.nlevels; ++l) sig = sig * 1099511628211ull + (uint64_t)pl.lv[l].K;
     return sig;
   }
@@ -412,6 +415,19 @@ struct Impl {
             hipLaunchKernelGGL((k_scan_up_team<Agg, R, F>), dim3((unsigned)s.nwg),
                                dim3(TeamCfg<Agg, R>::WGT), 0, st, (const R*)in, level_map(pl, l, 0),
                                out, tot, tm, f);
+            launched = true;
+          }
+        }
+        if constexpr (TEAM) {
+          if (l == 0 && pl.quad) {
+            if (s.wg == WG / 2)
+              hipLaunchKernelGGL((k_scan_up_quad<Agg, R, WG / 2, F>), dim3((unsigned)s.nwg),
+                                 dim3(WG / 2 * kQuadLanes), 0, st, (const R*)in,
+                                 level_map(pl, l, 0), out, tot, tm, f);
+            else
+              hipLaunchKernelGGL((k_scan_up_quad<Agg, R, WG, F>), dim3((unsigned)s.nwg),
+                                 dim3(WG * kQuadLanes), 0, st, (const R*)in, level_map(pl, l, 0),
+                                 out, tot, tm, f);
             launched = true;
           }
         }

@@ -39,6 +39,7 @@
 #include "psm_core.hpp"
 #include "psm_scan_mem.hpp"
 #include "psm_team.hpp"
+#include "psm_quad.hpp"
 
 namespace psm {
 
@@ -522,6 +523,77 @@ __global__ __launch_bounds__((TeamCfg<Agg, R>::WGT)) void k_scan_down_team(
   const V o{dst, lm.stride, item};
   if (w == 0) team_copy(e, o, r);
   else team_combine<R, NX>(V{const_cast<R*>(upper), um.stride, um.at(w)}, e, o, r, my, fb);
+}
+
+// Quad version of k_scan_up for level 0 of the square-root scans: VL quads
+// per workgroup; each quad is one virtual lane of the scan above (folds K
+// items, joins the Kogge-Stone scan over the quads, rewrites its items'
+// exclusive prefixes) and runs every combination cooperatively.  The level
+// layout is that of k_scan_up with VL lanes, so the two are interchangeable.
+template <typename Agg, typename R, int VL, typename FB = Fallback>
+__global__ __launch_bounds__(VL * kQuadLanes) void k_scan_up_quad(
+    const R* __restrict__ in, LevelMap lm, R* __restrict__ out_excl, R* __restrict__ totals,
+    LevelMap tm, FB fb) {
+  if (fb.skip()) return;
+  using V = typename ViewOf<Agg>::type;
+  constexpr int NX = Agg_nx<Agg>::value;
+  constexpr int NQ = Agg::NQ;
+  // buffers 0 and 1: running / scanned aggregates; buffer 2: the quad's next
+  // input item, copied from HBM before it is combined
+  __shared__ R lds[3 * NQ * VL];
+  const int t = (int)threadIdx.x / kQuadLanes;
+  const int r = (int)threadIdx.x % kQuadLanes;
+  auto slot = [&](int buf, int lane) { return V{lds + buf * NQ * VL, (int64_t)VL, (int64_t)lane}; };
+  R* inm = const_cast<R*>(in);
+  const V stage = slot(2, t);
+  const int K = lm.K;
+  const int64_t n = lm.n;
+  const int64_t base = ((int64_t)blockIdx.x * VL + t) * K;
+  int cur = 0;
+  if (base < n) quad_copy(V{inm, lm.stride, lm.at(base)}, slot(0, t), r);
+  else quad_identity(slot(0, t), r);
+  for (int j = 1; j < K; ++j) {
+    const int64_t p = base + j;
+    if (p >= n) break;
+    team_sync();
+    quad_copy(V{inm, lm.stride, lm.at(p)}, stage, r);
+    quad_combine<R, NX>(slot(cur, t), stage, slot(1 - cur, t), r, fb);
+    cur = 1 - cur;
+  }
+  if (cur == 1) {
+    team_sync();
+    quad_copy(slot(1, t), slot(0, t), r);
+  }
+  cur = 0;
+  for (int d = 1; d < VL; d <<= 1) {
+    __syncthreads();
+    if (t >= d) quad_combine<R, NX>(slot(cur, t - d), slot(cur, t), slot(1 - cur, t), r, fb);
+    else quad_copy(slot(cur, t), slot(1 - cur, t), r);
+    cur = 1 - cur;
+  }
+  __syncthreads();
+  if (t == VL - 1 && totals != nullptr)
+    quad_copy(slot(cur, t), V{totals, tm.stride, tm.at((int64_t)blockIdx.x)}, r);
+  // running exclusive prefix of this quad: run = slot(1 - cur, t); once every
+  // quad has read its predecessor's inclusive prefix, slot(cur, t) is free
+  V run = slot(1 - cur, t);
+  if (t == 0) quad_identity(run, r);
+  else quad_copy(slot(cur, t - 1), run, r);
+  __syncthreads();
+  V tmp = slot(cur, t);
+  for (int j = 0; j < K; ++j) {
+    const int64_t p = base + j;
+    if (p >= n) break;
+    const int64_t item = lm.at(p);
+    team_sync();
+    quad_copy(run, V{out_excl, lm.stride, item}, r);
+    if (j + 1 == K || p + 1 == n) break;  // the quad's inclusive total is not needed
+    quad_copy(V{inm, lm.stride, item}, stage, r);
+    quad_combine<R, NX>(run, stage, tmp, r, fb);
+    const V x = run;
+    run = tmp;
+    tmp = x;
+  }
 }
 
 // ============================================================================

@@ -81,10 +81,13 @@ class Problem:
         self._ws = None
 
     # candidate schedules of the library's autotuner: chunk, and scan fan-in
-    # with or without the team-cooperative upper scan levels (N.SCAN_TEAM) and
-    # the half-wave level 0 (N.SCAN_HALF)
+    # with or without the team-cooperative upper scan levels (N.SCAN_TEAM), the
+    # half-wave level 0 (N.SCAN_HALF) and the quad-cooperative level 0
+    # (N.SCAN_QUAD, with team upper levels)
     TUNE_CHUNKS = (2, 4, 8, 16, 32)
-    TUNE_SCAN_K = tuple(k | f for f in (0, N.SCAN_TEAM, N.SCAN_HALF, N.SCAN_TEAM | N.SCAN_HALF)
+    TUNE_SCAN_K = tuple(k | f for f in (0, N.SCAN_TEAM, N.SCAN_HALF, N.SCAN_TEAM | N.SCAN_HALF,
+                                        N.SCAN_TEAM | N.SCAN_QUAD,
+                                        N.SCAN_TEAM | N.SCAN_QUAD | N.SCAN_HALF)
                         for k in (2, 4, 8))
 
     def set_schedule(self, chunk: int, scan_k: int):

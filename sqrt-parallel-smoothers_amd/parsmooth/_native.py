@@ -15,7 +15,7 @@ import torch
 
 PSM_F32, PSM_F64 = 0, 1
 PSM_F32_TANGENT, PSM_F64_TANGENT = 2, 3
-SCAN_K_MASK, SCAN_TEAM, SCAN_HALF = 0xFF, 0x100, 0x200   # psm_problem_t.scan_k bits
+SCAN_K_MASK, SCAN_TEAM, SCAN_HALF, SCAN_QUAD = 0xFF, 0x100, 0x200, 0x400   # psm_problem_t.scan_k bits
 SCAN_TOP_SHIFT, SCAN_TOP_MASK = 16, 0xFF0000
 FORM_SQRT, FORM_STD = 0, 1
 

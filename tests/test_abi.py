@@ -25,7 +25,7 @@ def test_header_constants_match_binding():
     txt = open(os.path.join(ROOT, "include", "psm.h")).read()
     defs = {k: int(v, 0) for k, v in re.findall(r"#define (PSM_[A-Z0-9_]+) (0x[0-9a-fA-F]+|\d+)", txt)}
     assert defs["PSM_SCAN_K_MASK"] == N.SCAN_K_MASK and defs["PSM_SCAN_TEAM"] == N.SCAN_TEAM
-    assert defs["PSM_SCAN_HALF"] == N.SCAN_HALF
+    assert defs["PSM_SCAN_HALF"] == N.SCAN_HALF and defs["PSM_SCAN_QUAD"] == N.SCAN_QUAD
     assert defs["PSM_SCAN_TOP_SHIFT"] == N.SCAN_TOP_SHIFT and defs["PSM_SCAN_TOP_MASK"] == N.SCAN_TOP_MASK
     assert defs["PSM_F32_TANGENT"] == N.PSM_F32_TANGENT and defs["PSM_F64_TANGENT"] == N.PSM_F64_TANGENT
 

@@ -550,9 +550,10 @@ def test_every_schedule_matches_oracle(chunk):
     lengths = _SCHED_LENGTHS
     refs = _schedule_refs()
     top16 = 16 << N.SCAN_TOP_SHIFT  # top level may fold up to 16 items per lane
-    half = N.SCAN_HALF
+    half, quad = N.SCAN_HALF, N.SCAN_TEAM | N.SCAN_QUAD
     for k in (2, 4, 8, 2 | N.SCAN_TEAM, 8 | N.SCAN_TEAM, 2 | N.SCAN_TEAM | top16, 4 | top16,
-              2 | half, 4 | N.SCAN_TEAM | half, 2 | N.SCAN_TEAM | half):
+              2 | half, 4 | N.SCAN_TEAM | half, 2 | N.SCAN_TEAM | half, 2 | quad, 4 | quad | half,
+              2 | quad | half, 8 | N.SCAN_QUAD):
         r, obs, m0, nom = _schedule_runner(lengths, 3, chunk, k)
         assert r.chunk_size() == chunk
         out = r.prob.new_traj()
@@ -571,7 +572,8 @@ def test_deep_scan_schedules_agree():
     outs = []
     top16 = 16 << N.SCAN_TOP_SHIFT
     for chunk, k in ((32, 8), (4, 4), (2, 2), (4, 4 | N.SCAN_TEAM), (2, 2 | N.SCAN_TEAM),
-                     (8, 2 | N.SCAN_TEAM | top16), (2, 2 | top16), (4, 2 | N.SCAN_TEAM | N.SCAN_HALF)):
+                     (8, 2 | N.SCAN_TEAM | top16), (2, 2 | top16), (4, 2 | N.SCAN_TEAM | N.SCAN_HALF),
+                     (5, 2 | N.SCAN_TEAM | N.SCAN_QUAD | N.SCAN_HALF), (4, 4 | N.SCAN_TEAM | N.SCAN_QUAD)):
         r, _, _, nom = _schedule_runner([T], 4, chunk, k)
         out = r.prob.new_traj()
         r.pass_once((t64(nom[0]), t64(nom[1])), out)
@@ -725,7 +727,8 @@ def test_fp32_sigma_regression_small_noise(method, scale):
     assert np.all(np.abs(dg - dr) <= 1e-3 * dr + 1e-9), "fp32 regression noise factor diagonal"
 
 
-@pytest.mark.parametrize("k", [2, 2 | N.SCAN_TEAM, 2 | N.SCAN_TEAM | N.SCAN_HALF, 4 | N.SCAN_HALF])
+@pytest.mark.parametrize("k", [2, 2 | N.SCAN_TEAM, 2 | N.SCAN_TEAM | N.SCAN_HALF, 4 | N.SCAN_HALF,
+                               2 | N.SCAN_TEAM | N.SCAN_QUAD | N.SCAN_HALF, 4 | N.SCAN_QUAD])
 def test_fp32_schedules_match_oracle(k):
     """FP32 filter-smoother pass over a ragged batch with per-lane, team-cooperative
     and half-wave scan schedules, against the FP64 oracle run on the same
@@ -759,12 +762,14 @@ def test_fp32_schedules_match_oracle(k):
         close(gram(npy(got_L[s])), gram(ref.chol), tol=TOL32_COV, what=f"fp32 K {k:#x} seq {s} cov")
 
 
-def test_fp32_fallback_recomputes_in_fp64():
+@pytest.mark.parametrize("scan_k", [0, 2 | N.SCAN_TEAM | N.SCAN_QUAD | N.SCAN_HALF])
+def test_fp32_fallback_recomputes_in_fp64(scan_k):
     """FP32 square-root pass at an identical nominal.  With the re-triangularisation
     threshold forced to 1 every check fires and the FP64 recomputation path runs
     (counter > 0); with the check off nothing is counted.  Both results match the
     FP64 oracle evaluated on the same FP32-rounded inputs to the FP32 tolerance,
-    and recomputing in FP64 does not make the result worse."""
+    and recomputing in FP64 does not make the result worse.  The default schedule
+    and one with the quad-cooperative level 0."""
     from parsmooth import set_fp32_fallback
     from parsmooth.batch import IteratedSmoother
     T = 1500
@@ -787,6 +792,7 @@ def test_fp32_fallback_recomputes_in_fp64():
         for ratio in (0.0, 1.0):
             set_fp32_fallback(ratio)
             r = IteratedSmoother([T], td, od, extended, dtype=torch.float32)
+            r.prob.set_schedule(0, scan_k)
             r.set_data(t32(ys), t32(m0[None]), t32(np.eye(5)[None]))
             out = r.prob.new_traj()
             r.pass_once((t32(nom_m), t32(nom_c)), out)
