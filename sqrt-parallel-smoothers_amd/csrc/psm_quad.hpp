@@ -12,10 +12,10 @@
 // with DPP quad_perm (a register move); every lane then reflects all of its
 // rows, up to three independent row updates in flight per lane.
 //
-// Every step mirrors the per-lane operators (psm_scan_mem.hpp, tria_inplace in
-// psm_linalg.hpp) operation for operation -- the same reflector, the same fma
-// chains in the same order -- so a quad combination gives the per-lane
-// combination's result.  The code is branch-free between the operand reads
+// The steps mirror the per-lane operators (psm_scan_mem.hpp, tria_inplace in
+// psm_linalg.hpp) -- the same reflector, the same fma chains in the same
+// order -- except that A and b are formed as (A2 P) A1 and (A2 P) v rather
+// than A2 (P A1) and A2 (P v), which saves broadcasting P A1.  The code is branch-free between the operand reads
 // and the stores: a DPP broadcast reads the source lane's register, so the
 // four lanes of a quad execute every broadcast together.  Quads of one
 // wavefront may take different paths, since quad_perm never crosses quads.
@@ -197,7 +197,7 @@ PSM_DEV bool quad_ill(const C (&y)[QRows<NX>::NS][NC], float thr) {
 //   Xi = tria([[U1^T Z2, I], [Z2, 0]]),  M1 = U1 Xi11^{-T},  P = I - M1 Xi21^T
 //   v = b1 + U1 U1^T eta2,  w = eta2 - Z2 Z2^T b1
 //   eta = A1^T P^T w + eta1,  Z = tria([A1^T Xi22 | Z1])
-//   A = A2 (P A1),  b = A2 (P v) + b2,  U = tria([A2 M1 | U2])
+//   A = (A2 P) A1,  b = (A2 P) v + b2,  U = tria([A2 M1 | U2])
 // The 2nx rows of Xi and the nx rows of every nx-row result are in quad
 // layout; the nx-vectors t1, t2, v, w and P^T w are formed on every lane.
 // Returns (when ILL) whether U is ill-conditioned at thr.
@@ -364,62 +364,49 @@ PSM_DEV bool quad_combine_core(const FView<R, NX>& e1, const FView<R, NX>& e2,
   });
   PSM_QUAD_PHASE();
 
-  // rows of PA1 = P A1 and Pv = P v, then on every lane
-  C pa1[SN][n], pv[SN];
-  static_for<0, SN>([&](auto sc) {
-    constexpr int s = decltype(sc)::value;
-#pragma unroll
-    for (int j = 0; j < n; ++j) {
-      C acc = C(0);
-#pragma unroll
-      for (int k = 0; k < n; ++k) acc = fma(p[s][k], (C)e1.A(k, j), acc);
-      pa1[s][j] = acc;
-    }
-    C a = C(0);
-#pragma unroll
-    for (int k = 0; k < n; ++k) a = fma(p[s][k], v[k], a);
-    pv[s] = a;
-  });
-  C pa1f[n][n], pvf[n], m1f[n][n];
-  {
-    C pvs[SN][1];
-    static_for<0, SN>([&](auto sc) { pvs[decltype(sc)::value][0] = pv[decltype(sc)::value]; });
-    static_for<0, n>([&](auto kc) {
-      constexpr int k = decltype(kc)::value;
-      pvf[k] = qget<n, k, 0>(pvs);
-      static_for<0, n>([&](auto jc) {
-        constexpr int j = decltype(jc)::value;
-        pa1f[k][j] = qget<n, k, j>(pa1);
-        m1f[k][j] = qget<n, k, j>(m1);
-      });
+  // M1 on every lane
+  C m1f[n][n];
+  static_for<0, n>([&](auto kc) {
+    constexpr int k = decltype(kc)::value;
+    static_for<0, n>([&](auto jc) {
+      constexpr int j = decltype(jc)::value;
+      m1f[k][j] = qget<n, k, j>(m1);
     });
-  }
+  });
   PSM_QUAD_PHASE();
 
-  // S = A2 M1, A = A2 PA1, b = A2 Pv + b2; U = tria([S | U2])
+  // u = A2(i, :) P; A(i, :) = u A1, b(i) = b2(i) + u v, S(i, :) = A2(i, :) M1;
+  // U = tria([S | U2])
   C uc[SN][n2];
   static_for<0, SN>([&](auto sc) {
     constexpr int s = decltype(sc)::value;
     const int i = LN::template row<s>(r);
     const bool mine = s < LN::D || r == 0;
-    C a2[n];
+    C a2[n], u[n];
 #pragma unroll
     for (int k = 0; k < n; ++k) a2[k] = (C)e2.A(i, k);
-    C b = (C)e2.b(i);
-#pragma unroll
-    for (int k = 0; k < n; ++k) b = fma(a2[k], pvf[k], b);
-    if (mine) out.b(i) = (R)b;
 #pragma unroll
     for (int j = 0; j < n; ++j) {
       C s1 = C(0), s2 = C(0);
 #pragma unroll
       for (int k = 0; k < n; ++k) {
         s1 = fma(a2[k], m1f[k][j], s1);
-        s2 = fma(a2[k], pa1f[k][j], s2);
+        s2 = fma(a2[k], pf[k][j], s2);
       }
       uc[s][j] = s1;
-      if (mine) out.A(i, j) = (R)s2;
+      u[j] = s2;
       uc[s][n + j] = (C)e2.U(i, j);
+    }
+    C b = (C)e2.b(i);
+#pragma unroll
+    for (int k = 0; k < n; ++k) b = fma(u[k], v[k], b);
+    if (mine) out.b(i) = (R)b;
+#pragma unroll
+    for (int j = 0; j < n; ++j) {
+      C acc = C(0);
+#pragma unroll
+      for (int k = 0; k < n; ++k) acc = fma(u[k], (C)e1.A(k, j), acc);
+      if (mine) out.A(i, j) = (R)acc;
     }
   });
   PSM_QUAD_PHASE();
