@@ -210,8 +210,8 @@ PSM_DEV int n_params(int kind) {
 }
 
 // f and its Jacobian at x with a tangent on input j (j < NIN) or on model
-// parameter p (j = NIN + p): returns the tangents of f and of J.
-template <int NIN, int NOUT, typename S>
+// parameter p (j = NIN + p): returns the tangents of f and (WJ) of J.
+template <int NIN, int NOUT, bool WJ = true, typename S>
 PSM_DEV void model_tangent(const ModelView<D, S>& md, const D (&x)[NIN], int j, D (&df)[NOUT],
                            D (&dJ)[NOUT][NIN]) {
   using T = DualT<D>;
@@ -224,12 +224,12 @@ PSM_DEV void model_tangent(const ModelView<D, S>& md, const D (&x)[NIN], int j, 
   const int np = n_params<NIN, NOUT>(md.kind);
   for (int a = 0; a < MAXP; ++a) par[a] = T(a < np ? (D)md.params[a] : 0.0, (NIN + a == j) ? 1.0 : 0.0);
   ModelView<T, T> mt{md.kind, par, nullptr, nullptr, md.row, md.pt, md.npts};
-  model_eval<T, NIN, NOUT, true>(mt, xt, yt, Jt);
+  model_eval<T, NIN, NOUT, WJ>(mt, xt, yt, Jt);
 #pragma unroll
   for (int a = 0; a < NOUT; ++a) {
     df[a] = yt[a].d;
 #pragma unroll
-    for (int c = 0; c < NIN; ++c) dJ[a][c] = Jt[a][c].d;
+    for (int c = 0; c < NIN; ++c) dJ[a][c] = WJ ? Jt[a][c].d : 0.0;
   }
 }
 
@@ -390,8 +390,9 @@ PSM_DEV void lin_vjp(const ModelView<D, S>& md, const SigmaView<D>& sp, int use_
     }
   }
   // F = Psi^T P^-1:  Psi_bar = P^-1 Fbt^T,  P_bar = -P^-1 Psi Fbt P^-1 (symmetrised)
-  D Psib[NIN][NOUT], Pb[NIN][NIN];
+  D Psib[NIN][NOUT];
   {
+    D Pb[NIN][NIN];
     D FbT[NIN][NOUT], X[NIN][NOUT];
 #pragma unroll
     for (int i = 0; i < NIN; ++i)
@@ -410,6 +411,16 @@ PSM_DEV void lin_vjp(const ModelView<D, S>& md, const SigmaView<D>& sp, int use_
         Pb[i][j] = -s;
       }
     symmetrize(Pb);
+    // P = L L^T: L_bar += 2 P_bar L (lower part)
+#pragma unroll
+    for (int i = 0; i < NIN; ++i)
+#pragma unroll
+      for (int j = 0; j <= i; ++j) {
+        D s = 0.0;
+#pragma unroll
+        for (int k = 0; k < NIN; ++k) s = fma(Pb[i][k], L[k][j], s);
+        Lb[i][j] = fma(2.0, s, Lb[i][j]);
+      }
   }
   // pass B: f_bar_p = df_bar_p + wm_p m_f_bar with
   //   df_bar_p = rb_p + wc_p Psib^T dx_p, m_f_bar = bb - sum_p df_bar_p,
@@ -475,15 +486,6 @@ PSM_DEV void lin_vjp(const ModelView<D, S>& md, const SigmaView<D>& sp, int use_
 #pragma unroll
       for (int c = 0; c <= j; ++c) Lb[j][c] = fma(g, sp.pxi(p * NIN + c), Lb[j][c]);
     }
-    // parameters: <f_bar_p, df/dp> for the df_bar part
-    for (int q = 0; q < np; ++q) {
-      D dfp[NOUT], dJp[NOUT][NIN];
-      model_tangent<NIN, NOUT>(mp, x, NIN + q, dfp, dJp);
-      D s = 0.0;
-#pragma unroll
-      for (int i = 0; i < NOUT; ++i) s = fma(dfb[i], dfp[i], s);
-      pb[q] += s;
-    }
   }
   // pass C: the wm_p m_f_bar part of f_bar_p, m_f_bar = bb - sum_p df_bar_p
   D mfb[NOUT];
@@ -512,25 +514,53 @@ PSM_DEV void lin_vjp(const ModelView<D, S>& md, const SigmaView<D>& sp, int use_
 #pragma unroll
       for (int k = 0; k <= j; ++k) Lb[j][k] = fma(c, sp.pxi(p * NIN + k), Lb[j][k]);
     }
-    for (int q = 0; q < np; ++q) {
-      D dfp[NOUT], dJp[NOUT][NIN];
-      model_tangent<NIN, NOUT>(mp, x, NIN + q, dfp, dJp);
-      D s = 0.0;
+  }
+  // pass D: the model parameters, <f_bar_p, df(chi_p)/dp> with the whole
+  // f_bar_p = df_bar_p + wm_p m_f_bar (a pass of its own: the tangent
+  // evaluations need registers that passes B and C hold)
+  if (np > 0) {
+    for (int p = 0; p < npts; ++p) {
+      D dx[NIN], x[NIN], f[NOUT], r[NOUT];
 #pragma unroll
-      for (int i = 0; i < NOUT; ++i) s = fma(wm * mfb[i], dfp[i], s);
-      pb[q] += s;
+      for (int i = 0; i < NIN; ++i) {
+        D s = 0.0;
+#pragma unroll
+        for (int j = 0; j < NIN; ++j) s = fma(L[i][j], sp.pxi(p * NIN + j), s);
+        dx[i] = s;
+        x[i] = m[i] + s;
+      }
+      ModelView<D, S> mp = md;
+      mp.pt = p;
+      model_value<D, NIN, NOUT>(mp, x, f);
+#pragma unroll
+      for (int i = 0; i < NOUT; ++i) {
+        D s = f[i] - mf[i];
+#pragma unroll
+        for (int j = 0; j < NIN; ++j) s = fma(-F[i][j], dx[j], s);
+        r[i] = s;
+      }
+      const D wc = sp.pwc(p), wm = sp.pwm(p);
+      D fb[NOUT];
+#pragma unroll
+      for (int i = 0; i < NOUT; ++i) {
+        D s = 0.0;
+#pragma unroll
+        for (int k = 0; k < NOUT; ++k) s = fma(Ob[i][k], r[k], s);
+        D t = 2.0 * wc * s;
+#pragma unroll
+        for (int j = 0; j < NIN; ++j) t = fma(wc * Psib[j][i], dx[j], t);
+        fb[i] = fma(wm, mfb[i], t);
+      }
+      for (int q = 0; q < np; ++q) {
+        D dfp[NOUT], dJp[NOUT][NIN];
+        model_tangent<NIN, NOUT, false>(mp, x, NIN + q, dfp, dJp);
+        D s = 0.0;
+#pragma unroll
+        for (int i = 0; i < NOUT; ++i) s = fma(fb[i], dfp[i], s);
+        pb[q] += s;
+      }
     }
   }
-  // P = L L^T: L_bar += 2 P_bar L (lower part)
-#pragma unroll
-  for (int i = 0; i < NIN; ++i)
-#pragma unroll
-    for (int j = 0; j <= i; ++j) {
-      D s = 0.0;
-#pragma unroll
-      for (int k = 0; k < NIN; ++k) s = fma(Pb[i][k], L[k][j], s);
-      Lb[i][j] = fma(2.0, s, Lb[i][j]);
-    }
 }
 
 // Checkpoint components per state, each nx x nx: Abar_k = (I - K_k H_k) F_k
