@@ -309,9 +309,9 @@ int psm_iterated_smoother_graph(const psm_problem_t* prob, const psm_model_t* tr
  *
  * Workspace: caller-owned, psm_adjoint_workspace_bytes; psm_adjoint_prepare
  * writes the checkpoint there (the primal pass's filtered and smoothed
- * covariances, gains and filter-element matrices at the nominal, FP64), which
- * the VJP calls read, so it must stay intact until the last VJP of that
- * nominal.
+ * covariances, gains, filter-element matrices and linearized models at the
+ * nominal, FP64), which the VJP calls read, so it must stay intact until the
+ * last VJP of that nominal.
  * ------------------------------------------------------------------------- */
 #define PSM_ADJ_NTH(nx, ny) (3 * (nx) * (nx) + 2 * (nx) + (ny) * (nx) + (ny) + (ny) * (ny))
 

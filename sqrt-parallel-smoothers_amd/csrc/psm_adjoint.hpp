@@ -581,7 +581,9 @@ template <int NX> struct Pas {
                        WV = 2 * NX + 3 * S, WM = 3 * NX + 3 * S, N = 3 * NX + 4 * S;
 };
 // Cotangents of the linearized model per state: transition (used by step
-// k + 1) F, b, C and observation H, c, C.
+// k + 1) F, b, C and observation H, c, C.  The linearized model itself, the
+// same for every VJP at one fixed point, is kept in the same layout (computed
+// once by k_adj_prep).
 template <int NX, int NY> struct Lco {
   static constexpr int F = 0, B = NX * NX, CQ = NX * NX + NX, H = 2 * NX * NX + NX,
                        C = 2 * NX * NX + NX + NY * NX, CR = C + NY, N = CR + NY * NY;
@@ -596,8 +598,9 @@ __global__ __launch_bounds__(64) void k_adj_prep(int64_t M, int64_t Mp, const in
                                                  SigmaView<D> sp, int use_sigma, const R* __restrict__ nom_m,
                                                  const R* __restrict__ nom_L, const R* __restrict__ fm,
                                                  const R* __restrict__ fL, const R* __restrict__ sL,
-                                                 D* __restrict__ ck) {
+                                                 D* __restrict__ ck, D* __restrict__ ln) {
   using K = Ckp<NX>;
+  using LC = Lco<NX, NY>;
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= Mp) return;
   const int64_t i = state_of(t);
@@ -630,6 +633,9 @@ __global__ __launch_bounds__(64) void k_adj_prep(int64_t M, int64_t Mp, const in
     mt.row = i;
     lin_fwd<NX, NX>(mt, sp, use_sigma, m, Ln, F, C, b);
     stm(ck, Mp, t, K::FT, F);
+    stm(ln, Mp, t, LC::F, F);
+    stv(ln, Mp, t, LC::B, b);
+    stm(ln, Mp, t, LC::CQ, C);
     if (last) {
       stm(ck, Mp, t, K::G, zero);
     } else {
@@ -701,6 +707,9 @@ __global__ __launch_bounds__(64) void k_adj_prep(int64_t M, int64_t Mp, const in
     ModelView<D, R> mo = ob;
     mo.row = i;
     lin_fwd<NX, NY>(mo, sp, use_sigma, m, Ln, H, Cr, c);
+    stm2(ln, Mp, t, LC::H, H);
+    stv(ln, Mp, t, LC::C, c);
+    stm(ln, Mp, t, LC::CR, Cr);
     // K = Pp H^T S^-1, S = H Pp H^T + Cr Cr^T;  Abar = (I - K H) F
     D PH[NX][NY], Sm[NY][NY], Ls[NY][NY], KT[NY][NX], X[NY][NX];
 #pragma unroll
@@ -1028,8 +1037,8 @@ __global__ __launch_bounds__(64) void k_adj_chain(int64_t M, int64_t Mp, const i
                                                   const R* __restrict__ m0, const R* __restrict__ L0,
                                                   const R* __restrict__ nom_m, const R* __restrict__ nom_L,
                                                   const R* __restrict__ sm, const D* __restrict__ ck,
-                                                  const D* __restrict__ pa, D* __restrict__ lc,
-                                                  D* __restrict__ th, int64_t tst) {
+                                                  const D* __restrict__ ln, const D* __restrict__ pa,
+                                                  D* __restrict__ lc, D* __restrict__ th, int64_t tst) {
   using K = Ckp<NX>;
   using P = Pas<NX>;
   using LC = Lco<NX, NY>;
@@ -1101,9 +1110,9 @@ __global__ __launch_bounds__(64) void k_adj_chain(int64_t M, int64_t Mp, const i
     set_zero(Cb);
     if (!last) {
       D F[NX][NX], C[NX][NX], b[NX];
-      ModelView<D, R> mt = tr;
-      mt.row = i;
-      lin_fwd<NX, NX>(mt, sp, use_sigma, nm, nL, F, C, b);
+      ldm(ln, Mp, t, LC::F, F);
+      ldv(ln, Mp, t, LC::B, b);
+      ldm(ln, Mp, t, LC::CQ, C);
       const int64_t tn = pos_of(i + 1);
       D mn[NX];
       ld_cv(sm, i + 1, mn);
@@ -1235,9 +1244,9 @@ __global__ __launch_bounds__(64) void k_adj_chain(int64_t M, int64_t Mp, const i
     set_zero(Rb);
     if (!first) {
       D H[NY][NX], Cr[NY][NY], c[NY], y[NY], d[NY];
-      ModelView<D, R> mo = ob;
-      mo.row = i;
-      lin_fwd<NX, NY>(mo, sp, use_sigma, nm, nL, H, Cr, c);
+      ldm2(ln, Mp, t, LC::H, H);
+      ldv(ln, Mp, t, LC::C, c);
+      ldm(ln, Mp, t, LC::CR, Cr);
       ld_cv(ys, i - s - 1, y);
 #pragma unroll
       for (int r = 0; r < NY; ++r) d[r] = y[r] - c[r];
@@ -1324,7 +1333,8 @@ template <typename R, int NX, int NY, int US, int PART>
 __global__ __launch_bounds__(64) void k_adj_lin(int64_t M, int64_t Mp, const int64_t* __restrict__ soff,
                                                 int nseq, ModelView<D, R> tr, ModelView<D, R> ob, SigmaView<D> sp,
                                                 const R* __restrict__ nom_m, const R* __restrict__ nom_L,
-                                                const D* __restrict__ lc, const D* __restrict__ add_m,
+                                                const D* __restrict__ ln, const D* __restrict__ lc,
+                                                const D* __restrict__ add_m,
                                                 const D* __restrict__ add_L, D* __restrict__ out_m,
                                                 D* __restrict__ out_L, D* __restrict__ th, int64_t tst) {
   using LC = Lco<NX, NY>;
@@ -1347,10 +1357,10 @@ __global__ __launch_bounds__(64) void k_adj_lin(int64_t M, int64_t Mp, const int
     set_zero(qmb);
     set_zero(qcb);
     if (!last) {
-      D F[NX][NX], C[NX][NX], b[NX], Fb[NX][NX], bb[NX], Cb[NX][NX];
+      D C[NX][NX], Fb[NX][NX], bb[NX], Cb[NX][NX];
       ModelView<D, R> mt = tr;
       mt.row = i;
-      lin_fwd<NX, NX>(mt, sp, US, m, L, F, C, b);
+      ldm(ln, Mp, t, LC::CQ, C);
       ldm(lc, Mp, t, LC::F, Fb);
       ldv(lc, Mp, t, LC::B, bb);
       ldm(lc, Mp, t, LC::CQ, Cb);
@@ -1380,10 +1390,10 @@ __global__ __launch_bounds__(64) void k_adj_lin(int64_t M, int64_t Mp, const int
     set_zero(qmb);
     set_zero(qcb);
     if (!first) {
-      D H[NY][NX], C[NY][NY], c[NY], Hb[NY][NX], cb[NY], Cb[NY][NY];
+      D C[NY][NY], Hb[NY][NX], cb[NY], Cb[NY][NY];
       ModelView<D, R> mo = ob;
       mo.row = i;
-      lin_fwd<NX, NY>(mo, sp, US, m, L, H, C, c);
+      ldm(ln, Mp, t, LC::CR, C);
       ldm2(lc, Mp, t, LC::H, Hb);
       ldv(lc, Mp, t, LC::C, cb);
       ldm(lc, Mp, t, LC::CR, Cb);

@@ -705,7 +705,7 @@ struct Impl {
   static constexpr bool kAdj = SQRT && (std::is_same<R, float>::value || std::is_same<R, double>::value);
   struct AdjPlan {
     int64_t M = 0, Mp = 0;
-    size_t off_ck = 0, off_pa = 0, off_lc = 0, off_th = 0;
+    size_t off_ck = 0, off_pa = 0, off_lc = 0, off_ln = 0, off_th = 0;
     int nlv = 0;               // aggregate levels above the states
     int64_t lvn[16] = {}, lvst[16] = {};
     size_t off_lv[16] = {};
@@ -723,6 +723,8 @@ struct Impl {
     a.off_pa = off;
     off = align_up(off + (size_t)adj::Pas<NX>::N * a.Mp * d);
     a.off_lc = off;
+    off = align_up(off + (size_t)adj::Lco<NX, NY>::N * a.Mp * d);
+    a.off_ln = off;   // the linearized model at the fixed point, same layout
     off = align_up(off + (size_t)adj::Lco<NX, NY>::N * a.Mp * d);
     a.off_th = off;
     off = align_up(off + (size_t)adj::Th<NX, NY>::N * a.Mp * d);
@@ -787,7 +789,8 @@ struct Impl {
       TimedScope ts_("k_adj_prep", st);
       hipLaunchKernelGGL((adj::k_adj_prep<R, NX, NY>), dim3(grid_for(a.Mp, 64)), dim3(64), 0, st, a.M, a.Mp,
                          p->state_offsets, p->n_seq, trv, obv, sv, use, (const R*)nm, (const R*)nL,
-                         (const R*)fm, (const R*)fL, (const R*)sL, (double*)(aws + a.off_ck));
+                         (const R*)fm, (const R*)fL, (const R*)sL, (double*)(aws + a.off_ck),
+                         (double*)(aws + a.off_ln));
       return hip_check("k_adj_prep");
     }
     return PSM_OK;
@@ -855,6 +858,7 @@ struct Impl {
       const double* ck = (const double*)(aws + a.off_ck);
       double* pa = (double*)(aws + a.off_pa);
       double* lc = (double*)(aws + a.off_lc);
+      const double* ln = (const double*)(aws + a.off_ln);
       double* th = (double*)(aws + a.off_th);
       const unsigned g = grid_for(a.Mp, 64);
       if (mode == MODE_OUTPUT) {
@@ -874,13 +878,13 @@ struct Impl {
         if (mode == MODE_OUTPUT)
           hipLaunchKernelGGL((k_adj_chain<R, NX, NY, MODE_OUTPUT>), dim3(g), dim3(64), 0, st, a.M, a.Mp,
                              p->state_offsets, p->n_seq, trv, obv, sv, use, (const R*)ys, (const R*)m0,
-                             (const R*)L0, (const R*)nm, (const R*)nL, (const R*)sm, ck, (const double*)pa, lc,
-                             th, a.Mp);
+                             (const R*)L0, (const R*)nm, (const R*)nL, (const R*)sm, ck, ln, (const double*)pa,
+                             lc, th, a.Mp);
         else
           hipLaunchKernelGGL((k_adj_chain<R, NX, NY, MODE_FISHER>), dim3(g), dim3(64), 0, st, a.M, a.Mp,
                              p->state_offsets, p->n_seq, trv, obv, sv, use, (const R*)ys, (const R*)m0,
-                             (const R*)L0, (const R*)nm, (const R*)nL, (const R*)sm, ck, (const double*)pa, lc,
-                             th, a.Mp);
+                             (const R*)L0, (const R*)nm, (const R*)nL, (const R*)sm, ck, ln, (const double*)pa,
+                             lc, th, a.Mp);
         if ((rc = hip_check("k_adj_chain"))) return rc;
       }
       {
@@ -888,7 +892,7 @@ struct Impl {
         auto lin = [&](auto us, auto part) {
           hipLaunchKernelGGL((k_adj_lin<R, NX, NY, decltype(us)::value, decltype(part)::value>), dim3(g), dim3(64),
                              0, st, a.M, a.Mp, p->state_offsets, p->n_seq, trv, obv, sv, (const R*)nm,
-                             (const R*)nL, (const double*)lc, add_m, add_L, out_m, out_L, th, a.Mp);
+                             (const R*)nL, ln, (const double*)lc, add_m, add_L, out_m, out_L, th, a.Mp);
         };
         using I0 = std::integral_constant<int, 0>;
         using I1 = std::integral_constant<int, 1>;
