@@ -377,6 +377,10 @@ int psm_filter(const psm_problem_t* p, const psm_model_t* trans, const psm_model
   if ((rc = check_model(obs, p->nx, p->ny))) return rc;
   const OpsTable* t = need(p->dtype, p->form, p->nx, p->ny);
   if (!t) return PSM_E_UNSUPPORTED;
+  if (!ys || !m0 || !L0 || !nom_m || !nom_L || !out_m || !out_L) {
+    set_error("psm_filter: null array");
+    return PSM_E_ARG;
+  }
   return t->filter(p, trans, obs, sigma, ys, m0, L0, nom_m, nom_L, out_m, out_L, ell, ws, wsb,
                    (hipStream_t)stream, 0);
 }
@@ -389,6 +393,10 @@ int psm_smoother(const psm_problem_t* p, const psm_model_t* trans, const psm_sig
   if ((rc = check_model(trans, p->nx, p->nx))) return rc;
   const OpsTable* t = need(p->dtype, p->form, p->nx, p->ny);
   if (!t) return PSM_E_UNSUPPORTED;
+  if (!fm || !fL || !nom_m || !nom_L || !out_m || !out_L) {
+    set_error("psm_smoother: null array");
+    return PSM_E_ARG;
+  }
   return t->smoother(p, trans, sigma, fm, fL, nom_m, nom_L, out_m, out_L, ws, wsb,
                      (hipStream_t)stream, 0);
 }
@@ -404,6 +412,10 @@ int psm_filter_smoother(const psm_problem_t* p, const psm_model_t* trans, const 
   if ((rc = check_model(obs, p->nx, p->ny))) return rc;
   const OpsTable* t = need(p->dtype, p->form, p->nx, p->ny);
   if (!t) return PSM_E_UNSUPPORTED;
+  if (!ys || !m0 || !L0 || !nom_m || !nom_L || !filt_m || !filt_L || !out_m || !out_L) {
+    set_error("psm_filter_smoother: null array");
+    return PSM_E_ARG;
+  }
   auto run = [&](hipStream_t s) -> int {
     int r = t->filter(p, trans, obs, sigma, ys, m0, L0, nom_m, nom_L, filt_m, filt_L, nullptr, ws,
                       wsb, s, 0);
@@ -449,16 +461,22 @@ int psm_iterated_smoother_graph(const psm_problem_t* p, const psm_model_t* trans
   if (!t) return PSM_E_UNSUPPORTED;
   void* bm[2] = {out0_m, out1_m};
   void* bL[2] = {out0_L, out1_L};
+  // Between passes the filtered and smoothed trajectories stay in the
+  // workspace in the state-interleaved layout (null pointers select it), which
+  // the lane-per-chunk kernels read and write coalesced; the last pass writes
+  // its smoothed trajectory to out0 / out1.
   auto run = [&](hipStream_t s) -> int {
     const void* nm = init_m;
     const void* nL = init_L;
     for (int k = 0; k < n_iter; ++k) {
-      int r = t->filter(p, trans, obs, sigma, ys, m0, L0, nm, nL, filt_m, filt_L, nullptr, ws, wsb, s, 0);
+      const bool last = k + 1 == n_iter;
+      int r = t->filter(p, trans, obs, sigma, ys, m0, L0, nm, nL, nullptr, nullptr, nullptr, ws, wsb, s, 0);
       if (r) return r;
-      r = t->smoother(p, trans, sigma, filt_m, filt_L, nm, nL, bm[k & 1], bL[k & 1], ws, wsb, s, 1);
+      r = t->smoother(p, trans, sigma, nullptr, nullptr, nm, nL, last ? bm[k & 1] : nullptr,
+                      last ? bL[k & 1] : nullptr, ws, wsb, s, 1);
       if (r) return r;
-      nm = bm[k & 1];
-      nL = bL[k & 1];
+      nm = last ? bm[k & 1] : nullptr;
+      nL = last ? bL[k & 1] : nullptr;
     }
     if (ell != nullptr)
       return t->filter(p, trans, obs, sigma, ys, m0, L0, nm, nL, filt_m, filt_L, ell, ws, wsb, s, 0);
@@ -522,6 +540,10 @@ int psm_filter_smoother_noalloc(const psm_problem_t* p, const psm_model_t* trans
   if ((rc = check_model(obs, p->nx, p->ny))) return rc;
   const OpsTable* t = need(p->dtype, p->form, p->nx, p->ny);
   if (!t) return PSM_E_UNSUPPORTED;
+  if (!ys || !m0 || !L0 || !nom_m || !nom_L || !filt_m || !filt_L || !out_m || !out_L) {
+    set_error("psm_filter_smoother_noalloc: null array");
+    return PSM_E_ARG;
+  }
   const size_t need_bytes = t->workspace_bytes(p);
   if (need_bytes > (size_t)PSM_STATIC_WORKSPACE_BYTES) {
     set_error("psm_filter_smoother_noalloc: this problem needs " + std::to_string(need_bytes) +

@@ -61,7 +61,7 @@ struct Plan {
   int64_t nchunks = 0;
   int nlevels = 0;
   ScanLevel lv[16];
-  size_t off_lin = 0, off_ell = 0, off_selem = 0, off_guard = 0, total = 0;
+  size_t off_lin = 0, off_ell = 0, off_selem = 0, off_guard = 0, off_traj = 0, total = 0;
 };
 
 // Per-workgroup flag words of the guarded second launches of the FP32
@@ -252,6 +252,10 @@ struct Impl {
                         team ? kTeamVL : WG, team ? 2 : 4);
     pl.team = team;
     pl.quad = TEAM && (p->scan_k & PSM_SCAN_QUAD) != 0;
+    // one state-interleaved trajectory (mean and factor per state) for launch
+    // sequences that pass trajectories between their own passes
+    pl.off_traj = pl.total;
+    pl.total = align_up(pl.total + (size_t)(NX + NX * NX) * pl.Mp * sizeof(R));
     return pl;
   }
 
@@ -488,11 +492,14 @@ struct Impl {
     ModelView<R> obv = ob ? model_view<R>(ob) : model_view<R>(tr);
     trv.npts = use ? sv.npts : 0;
     obv.npts = use ? sv.npts : 0;
+    // nm == nullptr: the nominal is the workspace trajectory (state-interleaved)
+    const int inter = nm == nullptr ? 1 : 0;
+    if (inter) nm = ws + pl.off_traj;
     TimedScope ts_("k_linearize_states", st);
     launch_fb(fb, g, grid_for(pl.Mp, kLaneWG), [&](auto f) {
       hipLaunchKernelGGL((k_linearize_states<R, NX, NY, SQRT, decltype(f)>), dim3(grid_for(pl.Mp, kLaneWG)),
                          dim3(kLaneWG), 0, st, pl.M, pl.Mp, pl.chunk, trv, obv, ob ? 1 : 0, sv, use,
-                         (const R*)nm, (const R*)nL, (R*)(ws + pl.off_lin), f);
+                         (const R*)nm, (const R*)nL, inter, (R*)(ws + pl.off_lin), f);
     });
     return hip_check("k_linearize_states");
   }
@@ -516,6 +523,11 @@ struct Impl {
       if (rc) return rc;
     }
     const R* lin = (const R*)(ws + pl.off_lin);
+    // om == nullptr: the filtered trajectory goes to the workspace trajectory
+    // (state-interleaved), where a following smoother pass of the same launch
+    // sequence reads it
+    const int om_inter = om == nullptr ? 1 : 0;
+    if (om_inter) om = ws + pl.off_traj;
     {
     TimedScope ts_("k_filter_reduce", st);
     launch_fb(fb, g, grid_for(pl.nchunks, kLaneWG), [&](auto f) {
@@ -538,7 +550,7 @@ struct Impl {
                          (const R*)(ws + pre_off), level_map(pl, 0, 0),
                          up_off == SIZE_MAX ? nullptr : (const R*)(ws + up_off),
                          pl.nlevels > 1 ? level_map(pl, 1, 0) : level_map(pl, 0, 0),
-                         (R*)(ws + pl.lv[0].off_in), (R*)om, (R*)oL, ell_state, f);
+                         (R*)(ws + pl.lv[0].off_in), (R*)om, (R*)oL, om_inter, ell_state, f);
     });
     }
     if ((rc = hip_check("k_filter_apply"))) return rc;
@@ -567,13 +579,18 @@ struct Impl {
       rc = linearize_states(pl, tr, nullptr, sg, nm, nL, ws, st, fb, g);
       if (rc) return rc;
     }
+    // fm == nullptr: the filtered trajectory is the workspace trajectory; om ==
+    // nullptr: the smoothed one goes there (both state-interleaved)
+    const int fm_inter = fm == nullptr ? 1 : 0, om_inter = om == nullptr ? 1 : 0;
+    if (fm_inter) fm = ws + pl.off_traj;
+    if (om_inter) om = ws + pl.off_traj;
     {
     TimedScope ts_("k_smoother_reduce", st);
     launch_fb(fb, g, grid_for(pl.nchunks, kLaneWG), [&](auto f) {
       hipLaunchKernelGGL((k_smoother_reduce<R, NX, NY, SQRT, decltype(f)>),
                          dim3(grid_for(pl.nchunks, kLaneWG)), dim3(kLaneWG), 0, st, pl.M, pl.Mp, pl.chunk,
                          pl.nchunks, p->state_offsets, p->n_seq, (const R*)(ws + pl.off_lin), (const R*)fm,
-                         (const R*)fL, (R*)(ws + pl.off_selem), (R*)(ws + pl.lv[0].off_in),
+                         (const R*)fL, fm_inter, (R*)(ws + pl.off_selem), (R*)(ws + pl.lv[0].off_in),
                          level_map(pl, 0, 1), f);
     });
     }
@@ -588,7 +605,7 @@ struct Impl {
                            (const R*)(ws + pl.off_selem), (const R*)(ws + pre_off),
                            level_map(pl, 0, 1), up_off == SIZE_MAX ? nullptr : (const R*)(ws + up_off),
                            pl.nlevels > 1 ? level_map(pl, 1, 0) : level_map(pl, 0, 0),
-                           (R*)(ws + pl.lv[0].off_in), (R*)om, (R*)oL, f);
+                           (R*)(ws + pl.lv[0].off_in), (R*)om, (R*)oL, om_inter, f);
       });
     }
     return hip_check("k_smoother_apply");
@@ -635,7 +652,7 @@ struct Impl {
     launch_fb(fb, g, grid_for(pl.nchunks, kLaneWG), [&](auto f) {
       hipLaunchKernelGGL((k_smoother_reduce<R, NX, NY, SQRT, decltype(f)>), dim3(grid_for(pl.nchunks, kLaneWG)),
                          dim3(kLaneWG), 0, st, pl.M, pl.Mp, pl.chunk, pl.nchunks, p->state_offsets,
-                         p->n_seq, (const R*)(ws + pl.off_lin), (const R*)fm, (const R*)fL,
+                         p->n_seq, (const R*)(ws + pl.off_lin), (const R*)fm, (const R*)fL, 0,
                          (R*)(ws + pl.off_selem), (R*)(ws + pl.lv[0].off_in), level_map(pl, 0, 1), f);
     });
     if ((rc = hip_check("k_smoother_reduce"))) return rc;

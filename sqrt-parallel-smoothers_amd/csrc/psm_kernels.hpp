@@ -126,6 +126,38 @@ PSM_DEV void st_aos(R* __restrict__ p, int64_t i, const R (&v)[M][N]) {
     for (int b = 0; b < N; ++b) p[i * M * N + a * N + b] = v[a][b];
 }
 
+// Trajectories (a mean and a factor per state) come in two layouts: the API's
+// array-of-structs (state i at m + i NX, L + i NX^2; inter == 0), or, inside a
+// launch sequence that owns them, the state-interleaved layout of the
+// workspace (inter != 0: component q of state i at m[q Mp + pos], pos its
+// interleaved position, means first), which lane-per-chunk kernels read and
+// write coalesced.
+template <typename R, int NX>
+PSM_DEV void traj_load(const R* __restrict__ m_, const R* __restrict__ L_, int inter, int64_t Mp,
+                       int64_t i, int64_t pos, R (&m)[NX], R (&L)[NX][NX]) {
+  if (inter) {
+    int q = 0;
+    ld(m_, Mp, pos, q, m);
+    ld(m_, Mp, pos, q, L);
+  } else {
+    ld_aos(m_, i, m);
+    ld_aos(L_, i, L);
+  }
+}
+template <typename R, int NX>
+PSM_DEV void traj_store(R* __restrict__ m_, R* __restrict__ L_, int inter, int64_t Mp, int64_t i,
+                        int64_t pos, const R (&m)[NX], const R (&L)[NX][NX]) {
+  if (inter) {
+    int q = 0;
+    st(m_, Mp, pos, q, m);
+    st(m_, Mp, pos, q, L);
+  } else {
+    st_aos(m_, i, m);
+    st_aos(L_, i, L);
+  }
+}
+
+
 struct Lin {
   int use_sigma;
 };
@@ -249,8 +281,8 @@ PSM_DEV void linearize_point_w(const ModelView<R>& md, const SigmaView<R>& sp, i
 template <typename R, int NX, int NY, bool SQRT, typename FB = Fallback>
 __global__ __launch_bounds__(kLaneWG) void k_linearize_states(
     int64_t M, int64_t Mp, int C, ModelView<R> tr, ModelView<R> ob, int do_obs, SigmaView<R> sp,
-    int use_sigma, const R* __restrict__ nom_m, const R* __restrict__ nom_L, R* __restrict__ lin,
-    FB fb) {
+    int use_sigma, const R* __restrict__ nom_m, const R* __restrict__ nom_L, int nom_inter,
+    R* __restrict__ lin, FB fb) {
   if (fb.skip()) return;
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= Mp) return;
@@ -262,8 +294,7 @@ __global__ __launch_bounds__(kLaneWG) void k_linearize_states(
   tr.row = i;
   ob.row = i;
   R m[NX], L[NX][NX];
-  ld_aos(nom_m, i, m);
-  ld_aos(nom_L, i, L);
+  traj_load<R, NX>(nom_m, nom_L, nom_inter, Mp, i, t, m, L);
   {
     R F[NX][NX], Cq[NX][NX], b[NX];
     linearize_point_w<R, NX, NX, SQRT>(tr, sp, use_sigma, m, L, F, Cq, b, fb);
@@ -628,7 +659,7 @@ __global__ __launch_bounds__(kLaneWG) void k_filter_apply(
     const R* __restrict__ lin, const R* __restrict__ ys, const R* __restrict__ m0,
     const R* __restrict__ L0, const R* __restrict__ prefix, LevelMap lm,
     const R* __restrict__ upper, LevelMap um, R* __restrict__ joined, R* __restrict__ out_m,
-    R* __restrict__ out_L, typename AccOf<R>::type* __restrict__ ell_state, FB fb) {
+    R* __restrict__ out_L, int out_inter, typename AccOf<R>::type* __restrict__ ell_state, FB fb) {
   if (fb.skip()) return;
   using Acc = typename AccOf<R>::type;
   const StateMap sm{chunk};
@@ -660,8 +691,7 @@ __global__ __launch_bounds__(kLaneWG) void k_filter_apply(
       ld_aos(ys, i - s - 1, y);
       e = kf_step_fb<R, NX, NY, SQRT>(st_, y, m, L, fb);
     }
-    st_aos(out_m, i, m);
-    st_aos(out_L, i, L);
+    traj_store<R, NX>(out_m, out_L, out_inter, Mp, i, sm(i), m, L);
     if (ell_state != nullptr) ell_state[i] = e;
   }
 }
@@ -691,7 +721,7 @@ __global__ __launch_bounds__(BS) void k_ell_reduce(const Acc* __restrict__ ell_s
 template <typename R, int NX, int NY, bool SQRT, typename FB = Fallback>
 __global__ __launch_bounds__(kLaneWG) void k_smoother_reduce(
     int64_t M, int64_t Mp, int chunk, int64_t nchunks, const int64_t* __restrict__ soff, int nseq,
-    const R* __restrict__ lin, const R* __restrict__ fm, const R* __restrict__ fL,
+    const R* __restrict__ lin, const R* __restrict__ fm, const R* __restrict__ fL, int f_inter,
     R* __restrict__ elems, R* __restrict__ agg, LevelMap lm, FB fb) {
   if (fb.skip()) return;
   const StateMap sm{chunk};
@@ -705,8 +735,7 @@ __global__ __launch_bounds__(kLaneWG) void k_smoother_reduce(
   for (int64_t i = i1 - 1; i >= i0; --i) {
     while (i < soff[s]) --s;
     R m[NX], L[NX][NX];
-    ld_aos(fm, i, m);
-    ld_aos(fL, i, L);
+    traj_load<R, NX>(fm, fL, f_inter, Mp, i, sm(i), m, L);
     SAgg<R, NX> e;
     if (i == soff[s + 1] - 1) {
       copy(m, e.g);
@@ -733,7 +762,7 @@ template <typename R, int NX, bool SQRT, typename FB = Fallback>
 __global__ __launch_bounds__(kLaneWG) void k_smoother_apply(
     int64_t M, int64_t Mp, int chunk, int64_t nchunks, const R* __restrict__ elems,
     const R* __restrict__ prefix, LevelMap lm, const R* __restrict__ upper, LevelMap um,
-    R* __restrict__ joined, R* __restrict__ out_m, R* __restrict__ out_L, FB fb) {
+    R* __restrict__ joined, R* __restrict__ out_m, R* __restrict__ out_L, int out_inter, FB fb) {
   if (fb.skip()) return;
   const StateMap sm{chunk};
   const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -758,8 +787,7 @@ __global__ __launch_bounds__(kLaneWG) void k_smoother_apply(
     } else {
       smoother_apply_fb<R, NX, SQRT>(e, m, D, fb);
     }
-    st_aos(out_m, i, m);
-    st_aos(out_L, i, D);
+    traj_store<R, NX>(out_m, out_L, out_inter, Mp, i, sm(i), m, D);
   }
 }
 
