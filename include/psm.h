@@ -258,7 +258,7 @@ int psm_filter_smoother(const psm_problem_t* prob, const psm_model_t* trans,
  * device buffer of PSM_STATIC_WORKSPACE_BYTES that each GPU holds until the
  * process ends; problems whose psm_workspace_bytes exceeds it are rejected
  * with PSM_E_WORKSPACE (64 MiB holds a single FP64 nx=5 sequence of about
- * 55,000 steps).  Every call on one GPU shares that buffer: calls that
+ * 41,000 steps).  Every call on one GPU shares that buffer: calls that
  * overlap on one GPU (concurrent streams or threads) race and must be
  * serialised by the caller.  Nothing in it needs to start at zero. */
 int psm_filter_smoother_noalloc(const psm_problem_t* prob, const psm_model_t* trans,
@@ -271,7 +271,9 @@ int psm_filter_smoother_noalloc(const psm_problem_t* prob, const psm_model_t* tr
  * sequence: n_iter filter + smoother passes, pass k at the previous pass's
  * smoothed trajectory (pass 0 at init), then, when ell != NULL, the
  * log-likelihood filter pass at the final nominal (ell: n_seq doubles; the
- * filtered trajectory of that pass is left in filt_m / filt_L).  Replaces
+ * filtered trajectory of that pass is left in filt_m / filt_L, which nothing
+ * else writes).  Between passes the filtered and smoothed trajectories stay in
+ * the workspace, state-interleaved.  Replaces
  * parsmooth/methods.py:54-76 with criterion i < n_iter (the fixed_point
  * while_loop of parsmooth/_utils.py:136-146 run on the device).  The passes
  * write out0 and out1 alternately: the result is in out0 when n_iter is odd,
