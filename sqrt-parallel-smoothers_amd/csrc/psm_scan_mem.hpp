@@ -69,69 +69,69 @@ PSM_DEV void view_identity(const SView<R, NX>& d) {
 }
 
 // ---------------------------------------------------------------- square-root filter
-template <typename R, int NX>
+template <typename R, int NX, typename C = R>
 PSM_DEV void sqrt_combine_mem(const FView<R, NX>& e1, const FView<R, NX>& e2,
                               const FView<R, NX>& out) {
-  R X[2 * NX][2 * NX];
+  C X[2 * NX][2 * NX];
 #pragma unroll
   for (int i = 0; i < NX; ++i)
 #pragma unroll
     for (int j = 0; j < NX; ++j) {
-      R acc = R(0);
+      C acc = C(0);
 #pragma unroll
-      for (int k = 0; k < NX; ++k) acc = fma(e1.U(k, i), e2.Z(k, j), acc);
+      for (int k = 0; k < NX; ++k) acc = fma((C)e1.U(k, i), (C)e2.Z(k, j), acc);
       X[i][j] = acc;
-      X[i][NX + j] = (i == j) ? R(1) : R(0);
-      X[NX + i][j] = e2.Z(i, j);
-      X[NX + i][NX + j] = R(0);
+      X[i][NX + j] = (i == j) ? C(1) : C(0);
+      X[NX + i][j] = (C)e2.Z(i, j);
+      X[NX + i][NX + j] = C(0);
     }
-  tria_inplace<R, 2 * NX, 2 * NX>(X);
+  tria_inplace<C, 2 * NX, 2 * NX>(X);
   // M1 = U1 Xi11^{-T}
-  R M1[NX][NX];
+  C M1[NX][NX];
 #pragma unroll
   for (int j = 0; j < NX; ++j) {
-    const R inv = R(1) / X[j][j];
+    const C inv = C(1) / X[j][j];
 #pragma unroll
     for (int r = 0; r < NX; ++r) {
-      R s = e1.U(r, j);
+      C s = (C)e1.U(r, j);
 #pragma unroll
       for (int k = 0; k < j; ++k) s = fma(-M1[r][k], X[j][k], s);
       M1[r][j] = s * inv;
     }
   }
   // P = I - M1 Xi21^T
-  R Pm[NX][NX];
+  C Pm[NX][NX];
 #pragma unroll
   for (int i = 0; i < NX; ++i)
 #pragma unroll
     for (int j = 0; j < NX; ++j) {
-      R acc = (i == j) ? R(1) : R(0);
+      C acc = (i == j) ? C(1) : C(0);
 #pragma unroll
       for (int k = 0; k < NX; ++k) acc = fma(-M1[i][k], X[NX + j][k], acc);
       Pm[i][j] = acc;
     }
   // v = b1 + U1 (U1^T eta2) ; w = eta2 - Z2 (Z2^T b1)   (last reads of e2.eta, e2.Z)
-  R v[NX], w[NX];
+  C v[NX], w[NX];
   {
-    R t1[NX], t2[NX];
+    C t1[NX], t2[NX];
 #pragma unroll
     for (int j = 0; j < NX; ++j) {
-      R a1 = R(0), a2 = R(0);
+      C a1 = C(0), a2 = C(0);
 #pragma unroll
       for (int k = 0; k < NX; ++k) {
-        a1 = fma(e1.U(k, j), e2.eta(k), a1);
-        a2 = fma(e2.Z(k, j), e1.b(k), a2);
+        a1 = fma((C)e1.U(k, j), (C)e2.eta(k), a1);
+        a2 = fma((C)e2.Z(k, j), (C)e1.b(k), a2);
       }
       t1[j] = a1;
       t2[j] = a2;
     }
 #pragma unroll
     for (int i = 0; i < NX; ++i) {
-      R a1 = e1.b(i), a2 = e2.eta(i);
+      C a1 = (C)e1.b(i), a2 = (C)e2.eta(i);
 #pragma unroll
       for (int k = 0; k < NX; ++k) {
-        a1 = fma(e1.U(i, k), t1[k], a1);
-        a2 = fma(-e2.Z(i, k), t2[k], a2);
+        a1 = fma((C)e1.U(i, k), t1[k], a1);
+        a2 = fma(-(C)e2.Z(i, k), t2[k], a2);
       }
       v[i] = a1;
       w[i] = a2;
@@ -139,139 +139,139 @@ PSM_DEV void sqrt_combine_mem(const FView<R, NX>& e1, const FView<R, NX>& e2,
   }
   // eta = A1^T P^T w + eta1
   {
-    R PTw[NX];
+    C PTw[NX];
 #pragma unroll
     for (int j = 0; j < NX; ++j) {
-      R a = R(0);
+      C a = C(0);
 #pragma unroll
       for (int k = 0; k < NX; ++k) a = fma(Pm[k][j], w[k], a);
       PTw[j] = a;
     }
 #pragma unroll
     for (int i = 0; i < NX; ++i) {
-      R a = e1.eta(i);
+      C a = (C)e1.eta(i);
 #pragma unroll
-      for (int k = 0; k < NX; ++k) a = fma(e1.A(k, i), PTw[k], a);
-      out.eta(i) = a;
+      for (int k = 0; k < NX; ++k) a = fma((C)e1.A(k, i), PTw[k], a);
+      out.eta(i) = (R)(a);
     }
   }
   // Z = tria([A1^T Xi22 | Z1])
   {
-    R Zc[NX][2 * NX];
+    C Zc[NX][2 * NX];
 #pragma unroll
     for (int i = 0; i < NX; ++i)
 #pragma unroll
       for (int j = 0; j < NX; ++j) {
-        R acc = R(0);
+        C acc = C(0);
 #pragma unroll
-        for (int k = j; k < NX; ++k) acc = fma(e1.A(k, i), X[NX + k][NX + j], acc);
+        for (int k = j; k < NX; ++k) acc = fma((C)e1.A(k, i), X[NX + k][NX + j], acc);
         Zc[i][j] = acc;
-        Zc[i][NX + j] = e1.Z(i, j);
+        Zc[i][NX + j] = (C)e1.Z(i, j);
       }
-    tria_inplace<R, NX, 2 * NX>(Zc);
+    tria_inplace<C, NX, 2 * NX>(Zc);
 #pragma unroll
     for (int i = 0; i < NX; ++i)
 #pragma unroll
-      for (int j = 0; j < NX; ++j) out.Z(i, j) = Zc[i][j];
+      for (int j = 0; j < NX; ++j) out.Z(i, j) = (R)(Zc[i][j]);
   }
   // PA1 = P A1 ; Pv = P v
-  R PA1[NX][NX], Pv[NX];
+  C PA1[NX][NX], Pv[NX];
 #pragma unroll
   for (int i = 0; i < NX; ++i) {
 #pragma unroll
     for (int j = 0; j < NX; ++j) {
-      R acc = R(0);
+      C acc = C(0);
 #pragma unroll
-      for (int k = 0; k < NX; ++k) acc = fma(Pm[i][k], e1.A(k, j), acc);
+      for (int k = 0; k < NX; ++k) acc = fma(Pm[i][k], (C)e1.A(k, j), acc);
       PA1[i][j] = acc;
     }
-    R a = R(0);
+    C a = C(0);
 #pragma unroll
     for (int k = 0; k < NX; ++k) a = fma(Pm[i][k], v[k], a);
     Pv[i] = a;
   }
   // S = A2 M1 ; A = A2 PA1 ; b = A2 Pv + b2    (last reads of e2.A, e2.b)
-  R S[NX][NX];
+  C S[NX][NX];
 #pragma unroll
   for (int i = 0; i < NX; ++i) {
-    R a2[NX];
+    C a2[NX];
 #pragma unroll
-    for (int k = 0; k < NX; ++k) a2[k] = e2.A(i, k);
-    R bb = e2.b(i);
+    for (int k = 0; k < NX; ++k) a2[k] = (C)e2.A(i, k);
+    C bb = (C)e2.b(i);
 #pragma unroll
     for (int k = 0; k < NX; ++k) bb = fma(a2[k], Pv[k], bb);
 #pragma unroll
     for (int j = 0; j < NX; ++j) {
-      R s1 = R(0), s2 = R(0);
+      C s1 = C(0), s2 = C(0);
 #pragma unroll
       for (int k = 0; k < NX; ++k) {
         s1 = fma(a2[k], M1[k][j], s1);
         s2 = fma(a2[k], PA1[k][j], s2);
       }
       S[i][j] = s1;
-      out.A(i, j) = s2;
+      out.A(i, j) = (R)(s2);
     }
-    out.b(i) = bb;
+    out.b(i) = (R)(bb);
   }
   // U = tria([S | U2])
   {
-    R Uc[NX][2 * NX];
+    C Uc[NX][2 * NX];
 #pragma unroll
     for (int i = 0; i < NX; ++i)
 #pragma unroll
       for (int j = 0; j < NX; ++j) {
         Uc[i][j] = S[i][j];
-        Uc[i][NX + j] = e2.U(i, j);
+        Uc[i][NX + j] = (C)e2.U(i, j);
       }
-    tria_inplace<R, NX, 2 * NX>(Uc);
+    tria_inplace<C, NX, 2 * NX>(Uc);
 #pragma unroll
     for (int i = 0; i < NX; ++i)
 #pragma unroll
-      for (int j = 0; j < NX; ++j) out.U(i, j) = Uc[i][j];
+      for (int j = 0; j < NX; ++j) out.U(i, j) = (R)(Uc[i][j]);
   }
 }
 
 // ---------------------------------------------------------------- square-root smoother
 // out = e1 (+) e2 with e1 the later aggregate: g = E2 g1 + g2, E = E2 E1,
 // D = tria([E2 D1 | D2]).
-template <typename R, int NX>
+template <typename R, int NX, typename C = R>
 PSM_DEV void sqrt_smoother_combine_mem(const SView<R, NX>& e1, const SView<R, NX>& e2,
                                        const SView<R, NX>& out) {
-  R E2[NX][NX];
+  C E2[NX][NX];
 #pragma unroll
   for (int i = 0; i < NX; ++i)
 #pragma unroll
-    for (int j = 0; j < NX; ++j) E2[i][j] = e2.E(i, j);
-  R M[NX][2 * NX];
+    for (int j = 0; j < NX; ++j) E2[i][j] = (C)e2.E(i, j);
+  C M[NX][2 * NX];
 #pragma unroll
   for (int i = 0; i < NX; ++i)
 #pragma unroll
     for (int j = 0; j < NX; ++j) {
-      R acc = R(0);
+      C acc = C(0);
 #pragma unroll
-      for (int k = j; k < NX; ++k) acc = fma(E2[i][k], e1.D(k, j), acc);
+      for (int k = j; k < NX; ++k) acc = fma(E2[i][k], (C)e1.D(k, j), acc);
       M[i][j] = acc;
-      M[i][NX + j] = e2.D(i, j);
+      M[i][NX + j] = (C)e2.D(i, j);
     }
 #pragma unroll
   for (int i = 0; i < NX; ++i) {
-    R g = e2.g(i);
+    C g = (C)e2.g(i);
 #pragma unroll
-    for (int k = 0; k < NX; ++k) g = fma(E2[i][k], e1.g(k), g);
-    out.g(i) = g;
+    for (int k = 0; k < NX; ++k) g = fma(E2[i][k], (C)e1.g(k), g);
+    out.g(i) = (R)(g);
 #pragma unroll
     for (int j = 0; j < NX; ++j) {
-      R acc = R(0);
+      C acc = C(0);
 #pragma unroll
-      for (int k = 0; k < NX; ++k) acc = fma(E2[i][k], e1.E(k, j), acc);
-      out.E(i, j) = acc;
+      for (int k = 0; k < NX; ++k) acc = fma(E2[i][k], (C)e1.E(k, j), acc);
+      out.E(i, j) = (R)(acc);
     }
   }
-  tria_inplace<R, NX, 2 * NX>(M);
+  tria_inplace<C, NX, 2 * NX>(M);
 #pragma unroll
   for (int i = 0; i < NX; ++i)
 #pragma unroll
-    for (int j = 0; j < NX; ++j) out.D(i, j) = M[i][j];
+    for (int j = 0; j < NX; ++j) out.D(i, j) = (R)(M[i][j]);
 }
 
 }  // namespace psm
@@ -379,61 +379,6 @@ template <typename R, int NX> struct ViewOf<SAgg<R, NX>> { using type = SView<R,
 namespace psm {
 
 template <typename R, int NX>
-PSM_DEV void view_load_d(const FView<R, NX>& v, FAgg<double, NX>& a) {
-#pragma unroll
-  for (int i = 0; i < NX; ++i) {
-    a.b[i] = v.b(i);
-    a.eta[i] = v.eta(i);
-#pragma unroll
-    for (int j = 0; j < NX; ++j) {
-      a.A[i][j] = v.A(i, j);
-      a.U[i][j] = v.U(i, j);
-      a.Z[i][j] = v.Z(i, j);
-    }
-  }
-  a.flag = v.flag();
-}
-template <typename R, int NX>
-PSM_DEV void view_store_d(const FAgg<double, NX>& a, const FView<R, NX>& v) {
-#pragma unroll
-  for (int i = 0; i < NX; ++i) {
-    v.b(i) = (R)a.b[i];
-    v.eta(i) = (R)a.eta[i];
-#pragma unroll
-    for (int j = 0; j < NX; ++j) {
-      v.A(i, j) = (R)a.A[i][j];
-      v.U(i, j) = (R)a.U[i][j];
-      v.Z(i, j) = (R)a.Z[i][j];
-    }
-  }
-}
-template <typename R, int NX>
-PSM_DEV void view_load_d(const SView<R, NX>& v, SAgg<double, NX>& a) {
-#pragma unroll
-  for (int i = 0; i < NX; ++i) {
-    a.g[i] = v.g(i);
-#pragma unroll
-    for (int j = 0; j < NX; ++j) {
-      a.E[i][j] = v.E(i, j);
-      a.D[i][j] = v.D(i, j);
-    }
-  }
-  a.flag = v.flag();
-}
-template <typename R, int NX>
-PSM_DEV void view_store_d(const SAgg<double, NX>& a, const SView<R, NX>& v) {
-#pragma unroll
-  for (int i = 0; i < NX; ++i) {
-    v.g(i) = (R)a.g[i];
-#pragma unroll
-    for (int j = 0; j < NX; ++j) {
-      v.E(i, j) = (R)a.E[i][j];
-      v.D(i, j) = (R)a.D[i][j];
-    }
-  }
-}
-
-template <typename R, int NX>
 PSM_DEV bool view_ill(const FView<R, NX>& v, float thr) {
   R L[NX][NX];
 #pragma unroll
@@ -464,13 +409,12 @@ PSM_DEV void combine_mem_fb(const V& e1, const V& e2, const V& out, const FB& fb
   } else if constexpr (kFallback<R, SQRT>) {
     if (fb.thr > 0.f && f2 != FLAG_HEAD && f1 != FLAG_IDENT && f2 != FLAG_IDENT &&
         view_ill(out, fb.thr)) {
-      using AD = typename std::conditional<std::is_same<V, FView<R, NX>>::value, FAgg<double, NX>,
-                                           SAgg<double, NX>>::type;
-      AD a, b;
-      view_load_d(e1, a);
-      view_load_d(e2, b);
-      combine_into<double, NX, true>(a, b);
-      view_store_d(b, out);
+      // the memory-operand operator in FP64 on the FP32 operands: no aggregate
+      // is ever whole in registers
+      if constexpr (std::is_same<V, FView<R, NX>>::value)
+        sqrt_combine_mem<R, NX, double>(e1, e2, out);
+      else
+        sqrt_smoother_combine_mem<R, NX, double>(e1, e2, out);
       count_fallback(fb);
     }
   }
