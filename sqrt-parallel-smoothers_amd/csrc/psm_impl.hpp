@@ -375,10 +375,21 @@ struct Impl {
                          (int)g.words, 0u);
     return g;
   }
+  // waves: wavefronts of the launch.  A launch of at most PSM_FB_SINGLE_WAVES
+  // wavefronts (default 1024, one per SIMD of an MI355X) runs once with the
+  // FP64 code in place: its registers do not limit its occupancy, and the
+  // guarded second launch would only add a kernel node to a latency-bound
+  // pass.
+  static int64_t single_launch_waves() {
+    static const int64_t w = env_int("PSM_FB_SINGLE_WAVES", 1024, 0, 1 << 30);
+    return w;
+  }
   template <typename L>
-  static void launch_fb(const Fallback& fb, Guards& g, unsigned grid, L&& launch) {
+  static void launch_fb(const Fallback& fb, Guards& g, unsigned grid, int64_t waves, L&& launch) {
     if constexpr (kFallback<R, SQRT>) {
-      if ((int64_t)grid > g.words) {   // no flag word per workgroup: one launch with the FP64 code
+      // no flag word per workgroup, or a small launch with the check on: one
+      // launch with the FP64 code
+      if ((int64_t)grid > g.words || (fb.thr > 0.f && waves <= single_launch_waves())) {
         launch(fb);
         return;
       }
@@ -411,7 +422,11 @@ struct Impl {
       const LevelMap tm = top ? LevelMap{0, 0, 1, WG, 0} : level_map(pl, l + 1, 0);
       TimedScope ts_(l == 0 ? (filt ? "k_fscan_up_l0" : "k_sscan_up_l0")
                             : (filt ? "k_fscan_up" : "k_sscan_up"), st);
-      launch_fb(fb, g, (unsigned)s.nwg, [&](auto f) {
+      // wavefronts per workgroup of the kernel this level runs
+      const int64_t wpw = (TEAM && l > 0 && pl.team) ? TeamCfg<Agg, R>::WGT / 64
+                          : (TEAM && l == 0 && pl.quad) ? (s.wg * kQuadLanes + 63) / 64
+                                                        : (s.wg + 63) / 64;
+      launch_fb(fb, g, (unsigned)s.nwg, s.nwg * wpw, [&](auto f) {
         using F = decltype(f);
         bool launched = false;
         if constexpr (TEAM) {
@@ -455,7 +470,11 @@ struct Impl {
       const size_t upper_off =
           (l + 1 == pl.nlevels - 1) ? pl.lv[l + 1].off_out : pl.lv[l + 1].off_in;
       TimedScope ts_(filt ? "k_fscan_down" : "k_sscan_down", st);
-      launch_fb(fb, g, std::max((unsigned)(s.nwg * s.K), grid_for(s.nwg * s.K * s.wg, WG)), [&](auto f) {
+      const int64_t dwaves = (TEAM && l > 0 && pl.team)
+                                 ? s.nwg * s.K * (TeamCfg<Agg, R>::WGT / 64)
+                                 : (int64_t)grid_for(s.nwg * s.K * s.wg, WG) * ((WG + 63) / 64);
+      launch_fb(fb, g, std::max((unsigned)(s.nwg * s.K), grid_for(s.nwg * s.K * s.wg, WG)), dwaves,
+                [&](auto f) {
         using F = decltype(f);
         bool launched = false;
         if constexpr (TEAM) {
@@ -496,7 +515,7 @@ struct Impl {
     const int inter = nm == nullptr ? 1 : 0;
     if (inter) nm = ws + pl.off_traj;
     TimedScope ts_("k_linearize_states", st);
-    launch_fb(fb, g, grid_for(pl.Mp, kLaneWG), [&](auto f) {
+    launch_fb(fb, g, grid_for(pl.Mp, kLaneWG), grid_for(pl.Mp, kLaneWG), [&](auto f) {
       hipLaunchKernelGGL((k_linearize_states<R, NX, NY, SQRT, decltype(f)>), dim3(grid_for(pl.Mp, kLaneWG)),
                          dim3(kLaneWG), 0, st, pl.M, pl.Mp, pl.chunk, trv, obv, ob ? 1 : 0, sv, use,
                          (const R*)nm, (const R*)nL, inter, (R*)(ws + pl.off_lin), f);
@@ -530,7 +549,7 @@ struct Impl {
     if (om_inter) om = ws + pl.off_traj;
     {
     TimedScope ts_("k_filter_reduce", st);
-    launch_fb(fb, g, grid_for(pl.nchunks, kLaneWG), [&](auto f) {
+    launch_fb(fb, g, grid_for(pl.nchunks, kLaneWG), grid_for(pl.nchunks, kLaneWG), [&](auto f) {
       hipLaunchKernelGGL((k_filter_reduce<R, NX, NY, SQRT, decltype(f)>), dim3(grid_for(pl.nchunks, kLaneWG)),
                          dim3(kLaneWG), 0, st, pl.M, pl.Mp, pl.chunk, pl.nchunks, p->state_offsets,
                          p->n_seq, lin, (const R*)ys, (const R*)m0, (const R*)L0,
@@ -543,7 +562,7 @@ struct Impl {
     Acc* ell_state = ell ? (Acc*)(ws + pl.off_ell) : nullptr;
     {
     TimedScope ts_("k_filter_apply", st);
-    launch_fb(fb, g, grid_for(pl.nchunks, kLaneWG), [&](auto f) {
+    launch_fb(fb, g, grid_for(pl.nchunks, kLaneWG), grid_for(pl.nchunks, kLaneWG), [&](auto f) {
       hipLaunchKernelGGL((k_filter_apply<R, NX, NY, SQRT, decltype(f)>), dim3(grid_for(pl.nchunks, kLaneWG)),
                          dim3(kLaneWG), 0, st, pl.M, pl.Mp, pl.chunk, pl.nchunks, p->state_offsets,
                          p->n_seq, lin, (const R*)ys, (const R*)m0, (const R*)L0,
@@ -586,7 +605,7 @@ struct Impl {
     if (om_inter) om = ws + pl.off_traj;
     {
     TimedScope ts_("k_smoother_reduce", st);
-    launch_fb(fb, g, grid_for(pl.nchunks, kLaneWG), [&](auto f) {
+    launch_fb(fb, g, grid_for(pl.nchunks, kLaneWG), grid_for(pl.nchunks, kLaneWG), [&](auto f) {
       hipLaunchKernelGGL((k_smoother_reduce<R, NX, NY, SQRT, decltype(f)>),
                          dim3(grid_for(pl.nchunks, kLaneWG)), dim3(kLaneWG), 0, st, pl.M, pl.Mp, pl.chunk,
                          pl.nchunks, p->state_offsets, p->n_seq, (const R*)(ws + pl.off_lin), (const R*)fm,
@@ -599,7 +618,7 @@ struct Impl {
     if ((rc = run_scan<SA>(pl, ws, fb, g, st, &pre_off, &up_off))) return rc;
     {
       TimedScope ts_("k_smoother_apply", st);
-      launch_fb(fb, g, grid_for(pl.nchunks, kLaneWG), [&](auto f) {
+      launch_fb(fb, g, grid_for(pl.nchunks, kLaneWG), grid_for(pl.nchunks, kLaneWG), [&](auto f) {
         hipLaunchKernelGGL((k_smoother_apply<R, NX, SQRT, decltype(f)>), dim3(grid_for(pl.nchunks, kLaneWG)),
                            dim3(kLaneWG), 0, st, pl.M, pl.Mp, pl.chunk, pl.nchunks,
                            (const R*)(ws + pl.off_selem), (const R*)(ws + pre_off),
@@ -649,7 +668,7 @@ struct Impl {
     int rc = linearize_states(pl, tr, nullptr, sg, nm, nL, ws, st, fb, g);
     if (rc) return rc;
     // smoother elements at every state (the chunk aggregates it also forms are not used)
-    launch_fb(fb, g, grid_for(pl.nchunks, kLaneWG), [&](auto f) {
+    launch_fb(fb, g, grid_for(pl.nchunks, kLaneWG), grid_for(pl.nchunks, kLaneWG), [&](auto f) {
       hipLaunchKernelGGL((k_smoother_reduce<R, NX, NY, SQRT, decltype(f)>), dim3(grid_for(pl.nchunks, kLaneWG)),
                          dim3(kLaneWG), 0, st, pl.M, pl.Mp, pl.chunk, pl.nchunks, p->state_offsets,
                          p->n_seq, (const R*)(ws + pl.off_lin), (const R*)fm, (const R*)fL, 0,
