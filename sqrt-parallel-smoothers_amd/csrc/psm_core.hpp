@@ -293,7 +293,7 @@ PSM_DEV void sqrt_absorb(FAgg<R, NX>& a, const Step<R, NX, NY>& s, const R (&y)[
   R w[NY], G[NY][NX];
 #pragma unroll
   for (int i = 0; i < NY; ++i) {
-    const R inv = R(1) / P[i][i];
+    const R inv = rcp_(P[i][i]);
     R v = y[i] - s.c[i];
 #pragma unroll
     for (int k = 0; k < NX; ++k) v = fma(-s.H[i][k], b1[k], v);
@@ -372,7 +372,7 @@ PSM_DEV void sqrt_combine_core(const FAgg<R, NX>& e1, const FAgg<R, NX>& e2, FAg
   R M1[NX][NX];
 #pragma unroll
   for (int j = 0; j < NX; ++j) {
-    const R inv = R(1) / X[j][j];
+    const R inv = rcp_(X[j][j]);
 #pragma unroll
     for (int r = 0; r < NX; ++r) {
       R s = e1.U[r][j];
@@ -515,7 +515,7 @@ PSM_DEV void sqrt_filter_element(const Step<R, NX, NY>& s, const R (&m)[NX], con
   R G[NY][NX], w1[NY], w2[NY];
 #pragma unroll
   for (int i = 0; i < NY; ++i) {
-    const R inv = R(1) / P[i][i];
+    const R inv = rcp_(P[i][i]);
     R v1 = y[i] - s.c[i], v2 = y[i] - s.c[i];
 #pragma unroll
     for (int k = 0; k < NX; ++k) {
@@ -595,7 +595,7 @@ PSM_DEV void sqrt_smoother_element(const R (&F)[NX][NX], const R (&cq)[NX][NX], 
 #pragma unroll
   for (int jj = 0; jj < NX; ++jj) {
     const int j = NX - 1 - jj;
-    const R inv = R(1) / X[j][j];
+    const R inv = rcp_(X[j][j]);
 #pragma unroll
     for (int r = 0; r < NX; ++r) {
       R s = X[NX + r][j];
@@ -698,7 +698,7 @@ PSM_DEV void solve_lower_T_v(const R (&L)[N][N], const R (&b)[N], R (&x)[N]) {
     R s = b[i];
 #pragma unroll
     for (int j = i + 1; j < N; ++j) s = fma(-L[j][i], x[j], s);
-    x[i] = s / L[i][i];
+    x[i] = s * rcp_(L[i][i]);
   }
 }
 

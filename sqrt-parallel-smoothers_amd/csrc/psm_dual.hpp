@@ -65,6 +65,7 @@ template <typename B> PSM_DEV DualT<B> sqrt_(DualT<B> a) {
   const B s = sqrt_(a.v);
   return {s, s > B(0) ? a.d / (B(2) * s) : B(0)};
 }
+template <typename B> PSM_DEV DualT<B> rcp_(DualT<B> a) { return DualT<B>(1.0) / a; }
 template <typename B> PSM_DEV DualT<B> abs_(DualT<B> a) { return a.v < B(0) ? -a : a; }
 template <typename B> PSM_DEV DualT<B> log_(DualT<B> a) { return {log_(a.v), a.d / a.v}; }
 template <typename B> PSM_DEV bool isfinite(DualT<B> a) { return isfinite(a.v) && isfinite(a.d); }

@@ -103,9 +103,9 @@ PSM_DEV void quad_tria(C (&x)[QRows<M>::NS][NC], int r) {
 #pragma unroll
     for (int j = i + 1; j < NC; ++j) sigma = fma(x[ps][j], x[ps][j], sigma);
     const C nrm = sqrt_(fma(alpha, alpha, sigma));
-    const C v0 = (alpha > C(0)) ? -sigma / (alpha + nrm) : alpha - nrm;
+    const C v0 = (alpha > C(0)) ? -sigma * rcp_(alpha + nrm) : alpha - nrm;
     const C vv = fma(v0, v0, sigma);
-    const C f = (vv > C(0)) ? C(2) / vv : C(0);
+    const C f = (vv > C(0)) ? C(2) * rcp_(vv) : C(0);
     C bv0 = v0, bf = f;
     C bx[NC];
 #pragma unroll
@@ -259,7 +259,7 @@ PSM_DEV bool quad_combine_core(const FView<R, NX>& e1, const FView<R, NX>& e2,
   C m1[SN][n], p[SN][n];
   static_for<0, n>([&](auto jc) {
     constexpr int j = decltype(jc)::value;
-    const C inv = C(1) / xi11[j][j];
+    const C inv = rcp_(xi11[j][j]);
     static_for<0, SN>([&](auto sc) {
       constexpr int s = decltype(sc)::value;
       const int i = LN::template row<s>(r);

@@ -90,7 +90,7 @@ PSM_DEV void sqrt_combine_mem(const FView<R, NX>& e1, const FView<R, NX>& e2,
   C M1[NX][NX];
 #pragma unroll
   for (int j = 0; j < NX; ++j) {
-    const C inv = C(1) / X[j][j];
+    const C inv = rcp_(X[j][j]);
 #pragma unroll
     for (int r = 0; r < NX; ++r) {
       C s = (C)e1.U(r, j);

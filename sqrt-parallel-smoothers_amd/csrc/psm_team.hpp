@@ -118,9 +118,9 @@ PSM_DEV void team_tria(C (&x)[NC], int me, bool grp0) {
 #pragma unroll
     for (int j = i + 1; j < NC; ++j) sigma = fma(x[j], x[j], sigma);
     const C nrm = sqrt_(fma(alpha, alpha, sigma));
-    const C v0 = (alpha > C(0)) ? -sigma / (alpha + nrm) : alpha - nrm;
+    const C v0 = (alpha > C(0)) ? -sigma * rcp_(alpha + nrm) : alpha - nrm;
     const C vv = fma(v0, v0, sigma);
-    const C f = (vv > C(0)) ? C(2) / vv : C(0);
+    const C f = (vv > C(0)) ? C(2) * rcp_(vv) : C(0);
     const C bv0 = bcast2<i, G0, G1>(v0, grp0);
     const C bf = bcast2<i, G0, G1>(f, grp0);
     C bx[NC];
@@ -193,7 +193,7 @@ PSM_DEV C team_combine_core(const FView<R, NX>& e1, const FView<R, NX>& e2,
       constexpr int k = decltype(kc)::value;
       s = fma(-m1[k], bcast<j>(x[k]), s);
     });
-    m1[j] = s / bcast<j>(x[j]);
+    m1[j] = s * rcp_(bcast<j>(x[j]));
   });
   static_for<0, n>([&](auto jc) {
     constexpr int j = decltype(jc)::value;
