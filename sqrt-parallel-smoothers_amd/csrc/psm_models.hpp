@@ -616,28 +616,32 @@ PSM_DEV bool linearize_sigma_centred(const ModelView<R, S>& md, const SigmaView<
     mp.pt = p;
     model_increment<R, NIN, NOUT>(mp, m, d, f0, f);
   };
-  R mf[NOUT];   // weighted mean of the increments
+  // one pass for the weighted mean of the increments and their
+  // cross-covariance with the displacements:
+  //   Psi = sum wc d (f - mf)^T = sum wc d f^T - (sum wc d) mf^T
+  // (sum wc d vanishes for symmetric rules; its term is kept for the others)
+  R mf[NOUT], Psi[NIN][NOUT], sd[NIN];
   set_zero(mf);
-  for (int p = 0; p < sp.npts; ++p) {
-    R d[NIN], f[NOUT];
-    point(p, d, f);
-    const R w = sp.pwm(p);
-#pragma unroll
-    for (int i = 0; i < NOUT; ++i) mf[i] = fma(w, f[i], mf[i]);
-  }
-  R Psi[NIN][NOUT];
   set_zero(Psi);
+  set_zero(sd);
   for (int p = 0; p < sp.npts; ++p) {
     R d[NIN], f[NOUT];
     point(p, d, f);
-    const R w = sp.pwc(p);
+    const R wm = sp.pwm(p), wc = sp.pwc(p);
+#pragma unroll
+    for (int i = 0; i < NOUT; ++i) mf[i] = fma(wm, f[i], mf[i]);
 #pragma unroll
     for (int i = 0; i < NIN; ++i) {
-      const R wx = w * d[i];
+      const R wx = wc * d[i];
+      sd[i] += wx;
 #pragma unroll
-      for (int j = 0; j < NOUT; ++j) Psi[i][j] = fma(wx, f[j] - mf[j], Psi[i][j]);
+      for (int j = 0; j < NOUT; ++j) Psi[i][j] = fma(wx, f[j], Psi[i][j]);
     }
   }
+#pragma unroll
+  for (int i = 0; i < NIN; ++i)
+#pragma unroll
+    for (int j = 0; j < NOUT; ++j) Psi[i][j] = fma(-sd[i], mf[j], Psi[i][j]);
   // F = (L^-T L^-1 Psi)^T
   {
     R X[NIN][NOUT], Y[NIN][NOUT];
