@@ -236,7 +236,8 @@ PSM_DEV void linearize_point_wide(const ModelView<R>& md, const SigmaView<R>& sp
 template <typename R, int NIN, int NOUT, bool SQRT, typename FB>
 PSM_DEV void linearize_point_w(const ModelView<R>& md, const SigmaView<R>& sp, int use_sigma,
                                const R (&m)[NIN], const R (&Lx)[NIN][NIN], R (&F)[NOUT][NIN],
-                               R (&C)[NOUT][NOUT], R (&b)[NOUT], const FB& fb) {
+                               R (&C)[NOUT][NOUT], R (&b)[NOUT], const FB& fb,
+                               const PointCache<R>& pc = PointCache<R>{nullptr, 0, 0}) {
   if constexpr (kFallback<R, SQRT> && !FB::kRedo) {
     // FP32 square-root pass without FP64 code: a point the centred regression
     // cannot take (conditional moments, a negative weight) or whose factor is
@@ -247,7 +248,7 @@ PSM_DEV void linearize_point_w(const ModelView<R>& md, const SigmaView<R>& sp, i
       return;
     }
     const bool done = md.kind != PSM_MODEL_CONDITIONAL &&
-                      linearize_sigma_centred<R, NIN, NOUT, true>(md, sp, m, Lx, F, C, b);
+                      linearize_sigma_centred<R, NIN, NOUT, true>(md, sp, m, Lx, F, C, b, pc);
     if (!done || (fb.thr > 0.0f && ill_conditioned(C, fb.thr))) fb.raise();
   } else if constexpr (IsSingle<R>::value) {
     if (!use_sigma) {
@@ -257,7 +258,7 @@ PSM_DEV void linearize_point_w(const ModelView<R>& md, const SigmaView<R>& sp, i
     bool done = false;
     if (md.kind != PSM_MODEL_CONDITIONAL) {
       if constexpr (SQRT) {
-        done = linearize_sigma_centred<R, NIN, NOUT, true>(md, sp, m, Lx, F, C, b);
+        done = linearize_sigma_centred<R, NIN, NOUT, true>(md, sp, m, Lx, F, C, b, pc);
         if constexpr (kFallback<R, SQRT>) {
           if (done && fb.thr > 0.0f && ill_conditioned(C, fb.thr)) {
             count_fallback(fb);
@@ -267,7 +268,7 @@ PSM_DEV void linearize_point_w(const ModelView<R>& md, const SigmaView<R>& sp, i
       } else {
         R Lc[NIN][NIN];
         cholesky(Lx, Lc);   // get_mvnsqrt (parsmooth/linearization/_common.py:9-14)
-        done = linearize_sigma_centred<R, NIN, NOUT, false>(md, sp, m, Lc, F, C, b);
+        done = linearize_sigma_centred<R, NIN, NOUT, false>(md, sp, m, Lc, F, C, b, pc);
       }
     }
     if (!done) linearize_point_wide<R, NIN, NOUT, SQRT>(md, sp, use_sigma, m, Lx, F, C, b);
@@ -295,9 +296,15 @@ __global__ __launch_bounds__(kLaneWG) void k_linearize_states(
   ob.row = i;
   R m[NX], L[NX][NX];
   traj_load<R, NX>(nom_m, nom_L, nom_inter, Mp, i, t, m, L);
+  // FP32: the regression's increments wait in LDS for its residual pass
+  PointCache<R> pc{nullptr, 0, 0};
+  if constexpr (std::is_same<R, float>::value) {
+    __shared__ float fcache[(2 * NX + 1) * NX * kLaneWG];
+    pc = PointCache<R>{fcache + threadIdx.x, kLaneWG, 2 * NX + 1};
+  }
   {
     R F[NX][NX], Cq[NX][NX], b[NX];
-    linearize_point_w<R, NX, NX, SQRT>(tr, sp, use_sigma, m, L, F, Cq, b, fb);
+    linearize_point_w<R, NX, NX, SQRT>(tr, sp, use_sigma, m, L, F, Cq, b, fb, pc);
     int q = 0;
     st(lin, Mp, t, q, F);
     st(lin, Mp, t, q, Cq);

@@ -591,14 +591,23 @@ PSM_DEV void model_increment(const ModelView<R, S>& md, const R (&m)[NIN], const
 // sigma-point spread.  The points enter as displacements d_p = L xi_p and
 // increments f(m + d_p) - f(m) (model_increment), so neither the difference
 // Phi - F P F^T nor the rounding of a large mean reaches the noise factor.
-// Three passes over the points (mean, cross-covariance, residual columns)
-// re-evaluate the increments instead of keeping npts values per lane.
+// Two passes over the points: the mean and the cross-covariance, then the
+// residual columns, which read the increments back from the caller's per-lane
+// scratch (pc) when it holds them, else evaluate them again.
 // Returns false, with nothing written, when some wc is negative.
 // ----------------------------------------------------------------------------
+// Per-lane scratch for the increments of one regression (value i of point p
+// at base[(p * NOUT + i) * stride]), used when there are at most cap points.
+template <typename R> struct PointCache {
+  R* base;
+  int stride;
+  int cap;
+};
+
 template <typename R, int NIN, int NOUT, bool SQRT, typename S>
 PSM_DEV bool linearize_sigma_centred(const ModelView<R, S>& md, const SigmaView<R>& sp, const R (&m)[NIN],
                                      const R (&L)[NIN][NIN], R (&F)[NOUT][NIN], R (&C)[NOUT][NOUT],
-                                     R (&b)[NOUT]) {
+                                     R (&b)[NOUT], const PointCache<R>& pc = PointCache<R>{nullptr, 0, 0}) {
   for (int p = 0; p < sp.npts; ++p)
     if (sp.wc[p] < 0.0) return false;
   R f0[NOUT];
@@ -624,9 +633,14 @@ PSM_DEV bool linearize_sigma_centred(const ModelView<R, S>& md, const SigmaView<
   set_zero(mf);
   set_zero(Psi);
   set_zero(sd);
+  const bool keep = pc.base != nullptr && sp.npts <= pc.cap;
   for (int p = 0; p < sp.npts; ++p) {
     R d[NIN], f[NOUT];
     point(p, d, f);
+    if (keep) {
+#pragma unroll
+      for (int i = 0; i < NOUT; ++i) pc.base[(p * NOUT + i) * pc.stride] = f[i];
+    }
     const R wm = sp.pwm(p), wc = sp.pwc(p);
 #pragma unroll
     for (int i = 0; i < NOUT; ++i) mf[i] = fma(wm, f[i], mf[i]);
@@ -662,7 +676,19 @@ PSM_DEV bool linearize_sigma_centred(const ModelView<R, S>& md, const SigmaView<
   // residual of point p
   auto residual = [&](int p, R (&r)[NOUT]) {
     R d[NIN], f[NOUT];
-    point(p, d, f);
+    if (keep) {
+#pragma unroll
+      for (int i = 0; i < NIN; ++i) {
+        R s = R(0);
+#pragma unroll
+        for (int j = 0; j < NIN; ++j) s = fma(L[i][j], sp.pxi(p * NIN + j), s);
+        d[i] = s;
+      }
+#pragma unroll
+      for (int i = 0; i < NOUT; ++i) f[i] = pc.base[(p * NOUT + i) * pc.stride];
+    } else {
+      point(p, d, f);
+    }
 #pragma unroll
     for (int i = 0; i < NOUT; ++i) {
       R s = f[i] - mf[i];
