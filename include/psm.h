@@ -143,7 +143,10 @@ typedef struct psm_problem {
 
 int psm_version(void);
 const char* psm_last_error(void);
-/* 1 if (dtype, form, nx, ny) has compiled kernels. */
+/* 1 if (dtype, form, nx, ny) has compiled kernels: nx 1..8 and ny 1..4 for
+ * PSM_F32 / PSM_F64 in both forms; the tangent dtypes only for the square-root
+ * slices listed in the tangent registry (nx <= 5).  The reference takes any
+ * size from the array shapes (parsmooth/parallel/_operators.py:43-77). */
 int psm_supported(int32_t dtype, int32_t form, int32_t nx, int32_t ny);
 
 /* ---------------------------------------------------------------------------

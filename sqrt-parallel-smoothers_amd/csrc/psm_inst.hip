@@ -9,3 +9,4 @@
 PSM_INSTANTIATE(PSM_FN(1), PSM_R, PSM_NX, 1, PSM_SQRT)
 PSM_INSTANTIATE(PSM_FN(2), PSM_R, PSM_NX, 2, PSM_SQRT)
 PSM_INSTANTIATE(PSM_FN(3), PSM_R, PSM_NX, 3, PSM_SQRT)
+PSM_INSTANTIATE(PSM_FN(4), PSM_R, PSM_NX, 4, PSM_SQRT)

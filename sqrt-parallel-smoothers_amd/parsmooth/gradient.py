@@ -105,12 +105,28 @@ class IteratedLoglikelihood:
         self.method = resolve(linearization_method)
         self.n_iter = int(n_iter)
         self.prob = Problem(nx, ny, lengths, dtype, True, self.dev)
-        tcode = N.PSM_F64_TANGENT if dtype == torch.float64 else N.PSM_F32_TANGENT
-        self.tprob = Problem(nx, ny, lengths, dtype, True, self.dev, tangent_code=tcode)
+        self.lengths = lengths
+        self._tprob = None
         self.ys = to_dev(observations, dtype, self.dev).reshape(self.prob.n_obs, ny)
-        self.ys_dual = _dual(self.ys, torch.zeros_like(self.ys))
+        self._ys_dual = None
         self.init = (to_dev(init_nominal[0], dtype, self.dev).reshape(self.prob.n_states, nx),
                      to_dev(init_nominal[1], dtype, self.dev).reshape(self.prob.n_states, nx, nx))
+
+    @property
+    def tprob(self):
+        """The tangent problem of forward mode, made on first use: the tangent
+        kernels exist for fewer sizes than the value kernels, and reverse mode
+        does not need them."""
+        if self._tprob is None:
+            tcode = N.PSM_F64_TANGENT if self.dtype == torch.float64 else N.PSM_F32_TANGENT
+            self._tprob = Problem(self.nx, self.ny, self.lengths, self.dtype, True, self.dev, tangent_code=tcode)
+        return self._tprob
+
+    @property
+    def ys_dual(self):
+        if self._ys_dual is None:
+            self._ys_dual = _dual(self.ys, torch.zeros_like(self.ys))
+        return self._ys_dual
 
     # -------------------------------------------------------------- passes
     def _sigma(self, tangent):

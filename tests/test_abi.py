@@ -41,10 +41,12 @@ def test_supported_slices():
     lib = N.load_library()
     for dtype in (N.PSM_F32, N.PSM_F64):
         for form in (N.FORM_SQRT, N.FORM_STD):
-            for nx in range(1, 6):
-                for ny in range(1, 4):
+            for nx in range(1, 9):
+                for ny in range(1, 5):
                     assert lib.psm_supported(dtype, form, nx, ny) == 1
-    assert lib.psm_supported(N.PSM_F64, N.FORM_SQRT, 6, 2) == 0
+    assert lib.psm_supported(N.PSM_F64, N.FORM_SQRT, 9, 2) == 0
+    assert lib.psm_supported(N.PSM_F64, N.FORM_SQRT, 5, 5) == 0
+    assert lib.psm_supported(N.PSM_F64, N.FORM_SQRT, 0, 1) == 0
     assert lib.psm_supported(7, N.FORM_SQRT, 5, 2) == 0
     # tangent (forward-mode) slices: square-root form only
     for nx, ny in ((1, 1), (2, 1), (2, 2), (3, 2), (4, 2), (5, 2)):
