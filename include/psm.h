@@ -374,6 +374,14 @@ int psm_sample(const psm_problem_t* prob, const psm_model_t* trans, const psm_si
 int psm_mean_sq_diff(int32_t dtype, int64_t n, const void* a, const void* b, double* out,
                      void* stream);
 
+/* Diagnostic, no reference counterpart (the reference divides and takes
+ * square roots through XLA): out_rcp[i] = 1 / x[i] and out_sqrt[i] = sqrt(x[i])
+ * for n FP64 device values, computed as every FP64 factorisation of this
+ * library computes them (hardware estimates refined in software, for normal
+ * finite arguments, and sqrt(0) = 0), so that tests can compare them with
+ * correctly rounded results. */
+int psm_math_check(int64_t n, const double* x, double* out_rcp, double* out_sqrt, void* stream);
+
 /* Schedule autotuner (no reference counterpart: the reference's schedule is
  * XLA's).  Times psm_filter_smoother (eager, on `stream`) for candidate
  * schedules whose workspace fits workspace_bytes: first chunk in {2, 4, 8, 16,

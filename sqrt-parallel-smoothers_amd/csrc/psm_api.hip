@@ -117,6 +117,17 @@ __global__ __launch_bounds__(1024) void k_mean_sq_diff(int64_t n, const R* __res
   if (threadIdx.x == 0) *out = n > 0 ? red[0] / (double)n : 0.0;
 }
 
+// ---------------------------------------------------------------- FP64 division / square root
+// The reciprocal and square root every FP64 factorisation uses (psm_linalg.hpp),
+// one value per lane, for the tests that compare them with correctly rounded results.
+__global__ __launch_bounds__(256) void k_math_check(int64_t n, const double* __restrict__ x,
+                                                    double* __restrict__ r, double* __restrict__ q) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  r[i] = rcp_(x[i]);
+  q[i] = sqrt_(x[i]);
+}
+
 // ---------------------------------------------------------------- graph cache
 // Captured launch sequences, keyed by everything the launches depend on (dims,
 // the resolved schedule, models, every buffer address, the caller's stream).
@@ -739,6 +750,22 @@ int psm_mean_sq_diff(int32_t dtype, int64_t n, const void* a, const void* b, dou
                        (const float*)b, out);
   else { set_error("bad dtype"); return PSM_E_ARG; }
   return hip_check("k_mean_sq_diff");
+}
+
+int psm_math_check(int64_t n, const double* x, double* out_rcp, double* out_sqrt, void* stream) {
+  if (n < 0 || (n > 0 && (!x || !out_rcp || !out_sqrt))) {
+    set_error("psm_math_check: bad arguments");
+    return PSM_E_ARG;
+  }
+  if (n == 0) return PSM_OK;
+  const int64_t blocks = (n + 255) / 256;
+  if (blocks > 0x7fffffff) {
+    set_error("psm_math_check: n too large");
+    return PSM_E_ARG;
+  }
+  hipLaunchKernelGGL(k_math_check, dim3((unsigned)blocks), dim3(256), 0, (hipStream_t)stream, n, x,
+                     out_rcp, out_sqrt);
+  return hip_check("k_math_check");
 }
 
 void psm_timing_enable(int32_t enable) {
