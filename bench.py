@@ -583,7 +583,7 @@ def perturbation_growth(runner, init, n_iter, seed=0):
     return out
 
 
-def north_star_smoothing(n_seq, T, n_iter, steps, warmup, seed=5):
+def smoothing_fp32_vs_fp64(n_seq, T, n_iter, steps, warmup, seed=5):
     """FP32 against FP64 square-root cubature IPLS on the gradient workload
     (n_seq coordinated-turn sequences of length T simulated at qc=0.01, qw=0.1,
     sigma=0.5, n_iter iterations from the simulated trajectories): smoothed
@@ -775,7 +775,7 @@ def main():
                     help="comma-separated log2 T of the FP32 cubature error report ('' to skip)")
     ap.add_argument("--stability", type=str, default="80,100,1000,7500",
                     help="T values of the FP32 robustness protocol ('all': the reference's 24; '': skip)")
-    ap.add_argument("--north-star-seqs", type=int, default=256,
+    ap.add_argument("--smoothing-seqs", type=int, default=256,
                     help="sequences of the FP32-vs-FP64 cubature IPLS smoothing leg (0 to skip)")
     ap.add_argument("--cpu-batch-seqs", type=int, default=384,
                     help="sequences of the all-core CPU baseline sample of the batch workload (0 to skip)")
@@ -905,10 +905,10 @@ def main():
                 "errors_over": "the sequences at a fixed point (summed gradient, summed ell)",
                 "grad_fp64": ref.tolist()}
 
-    north = None
-    if args.north_star_seqs > 0:
-        log(f"north-star smoothing leg: {args.north_star_seqs} x T={args.grad_T}, FP32 vs FP64 cubature IPLS")
-        north = north_star_smoothing(args.north_star_seqs, args.grad_T, args.grad_iter, max(1, args.steps // 2), 1)
+    smoothing = None
+    if args.smoothing_seqs > 0:
+        log(f"smoothing leg: {args.smoothing_seqs} x T={args.grad_T}, FP32 vs FP64 cubature IPLS")
+        smoothing = smoothing_fp32_vs_fp64(args.smoothing_seqs, args.grad_T, args.grad_iter, max(1, args.steps // 2), 1)
 
     sweep = lin = errs = stab = pub = wide = None
     if args.sweep:
@@ -978,7 +978,7 @@ def main():
         "fp32_ms_per_step": dt32 * 1e3, "fp32_over_fp64": dt64 / dt32,
         "fp32_rel_err_single_pass": rel32_pass, "fp32_rel_err_after_iterations": rel32_end,
         "fp64_growth_of_fp32_sized_perturbations": growth,
-        "north_star_fp32_vs_fp64": north,
+        "smoothing_fp32_vs_fp64": smoothing,
         "fp32_fp64_fallbacks_per_step": fb32,
         "alg_bytes_per_step_it": ALG_BYTES_WORDS * 8, "alg_flops_per_step_it": ALG_FLOPS,
         "kernel_ms": {k: v[0] / v[1] for k, v in rep.items()},
