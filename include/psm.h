@@ -374,6 +374,29 @@ int psm_sample(const psm_problem_t* prob, const psm_model_t* trans, const psm_si
 int psm_mean_sq_diff(int32_t dtype, int64_t n, const void* a, const void* b, double* out,
                      void* stream);
 
+/* Runtime-size helpers of parsmooth._utils (parsmooth/_utils.py), one lane per
+ * batch item, each item row-major and contiguous in device memory; PSM_F32 /
+ * PSM_F64.  They serve the Python utilities; the passes keep their blocks in
+ * registers at the compiled sizes.
+ * psm_tria replaces tria (_utils.py:22-24): A (nb x rows x cols) is
+ *   triangularised in place; its first min(rows, cols) columns become L (lower,
+ *   non-negative diagonal; the reference keeps QR's signs) with L L^T = A A^T,
+ *   the other columns zero.
+ * psm_cholesky_update replaces _cholesky_update and cholesky_update_many
+ *   (_utils.py:13-20, 39-81): L (nb x n x n, lower, in place) becomes the factor
+ *   of L L^T + multiplier v v^T for each of the nv vectors V (nb x nv x n) in
+ *   turn; after each vector the entries above the diagonal are dropped and
+ *   non-finite entries set to 0, as in the reference.  work: nb x n values of
+ *   the dtype.
+ * psm_mvn_loglikelihood replaces mvn_loglikelihood (_utils.py:152-160):
+ *   out[b] = log N(x_b; 0, L_b L_b^T) for x (nb x n) and lower L (nb x n x n),
+ *   in FP64.  work: nb x n doubles. */
+int psm_tria(int32_t dtype, int64_t nb, int32_t rows, int32_t cols, void* A, void* stream);
+int psm_cholesky_update(int32_t dtype, int64_t nb, int32_t n, int32_t nv, void* L, const void* V,
+                        double multiplier, void* work, void* stream);
+int psm_mvn_loglikelihood(int32_t dtype, int64_t nb, int32_t n, const void* x, const void* L,
+                          double* out, double* work, void* stream);
+
 /* Diagnostic, no reference counterpart (the reference divides and takes
  * square roots through XLA): out_rcp[i] = 1 / x[i] and out_sqrt[i] = sqrt(x[i])
  * for n FP64 device values, computed as every FP64 factorisation of this

@@ -14,6 +14,7 @@
 #include "psm.h"
 #include "psm_impl.hpp"
 #include "psm_registry.hpp"
+#include "psm_util.hpp"
 
 #define PSM_DECL(T, NX, NY) const psm::OpsTable* psm_ops_##T##_##NX##_##NY();
 PSM_FOR_ALL(PSM_DECL)
@@ -766,6 +767,67 @@ int psm_math_check(int64_t n, const double* x, double* out_rcp, double* out_sqrt
   hipLaunchKernelGGL(k_math_check, dim3((unsigned)blocks), dim3(256), 0, (hipStream_t)stream, n, x,
                      out_rcp, out_sqrt);
   return hip_check("k_math_check");
+}
+
+// Grid of one lane per batch item for the runtime-size helpers; 0 when nb is
+// out of range (the caller reports it).
+static unsigned util_grid(int64_t nb) {
+  const int64_t g = (nb + kUtilWG - 1) / kUtilWG;
+  return (g < 1 || g > 0x7fffffff) ? 0u : (unsigned)g;
+}
+
+int psm_tria(int32_t dtype, int64_t nb, int32_t rows, int32_t cols, void* A, void* stream) {
+  if (nb < 0 || rows < 1 || cols < 1 || (nb > 0 && !A) || (nb > 0 && util_grid(nb) == 0)) {
+    set_error("psm_tria: bad arguments");
+    return PSM_E_ARG;
+  }
+  if (nb == 0) return PSM_OK;
+  hipStream_t st = (hipStream_t)stream;
+  if (dtype == PSM_F64)
+    hipLaunchKernelGGL(k_util_tria<double>, dim3(util_grid(nb)), dim3(kUtilWG), 0, st, nb, rows, cols,
+                       (double*)A);
+  else if (dtype == PSM_F32)
+    hipLaunchKernelGGL(k_util_tria<float>, dim3(util_grid(nb)), dim3(kUtilWG), 0, st, nb, rows, cols,
+                       (float*)A);
+  else { set_error("psm_tria: bad dtype"); return PSM_E_ARG; }
+  return hip_check("k_util_tria");
+}
+
+int psm_cholesky_update(int32_t dtype, int64_t nb, int32_t n, int32_t nv, void* L, const void* V,
+                        double multiplier, void* work, void* stream) {
+  if (nb < 0 || n < 1 || nv < 0 || (nb > 0 && (!L || !work || (nv > 0 && !V))) ||
+      (nb > 0 && util_grid(nb) == 0)) {
+    set_error("psm_cholesky_update: bad arguments");
+    return PSM_E_ARG;
+  }
+  if (nb == 0 || nv == 0) return PSM_OK;
+  hipStream_t st = (hipStream_t)stream;
+  if (dtype == PSM_F64)
+    hipLaunchKernelGGL(k_util_chol_update<double>, dim3(util_grid(nb)), dim3(kUtilWG), 0, st, nb, n, nv,
+                       (double*)L, (const double*)V, (double*)work, multiplier);
+  else if (dtype == PSM_F32)
+    hipLaunchKernelGGL(k_util_chol_update<float>, dim3(util_grid(nb)), dim3(kUtilWG), 0, st, nb, n, nv,
+                       (float*)L, (const float*)V, (float*)work, (float)multiplier);
+  else { set_error("psm_cholesky_update: bad dtype"); return PSM_E_ARG; }
+  return hip_check("k_util_chol_update");
+}
+
+int psm_mvn_loglikelihood(int32_t dtype, int64_t nb, int32_t n, const void* x, const void* L,
+                          double* out, double* work, void* stream) {
+  if (nb < 0 || n < 1 || (nb > 0 && (!x || !L || !out || !work)) || (nb > 0 && util_grid(nb) == 0)) {
+    set_error("psm_mvn_loglikelihood: bad arguments");
+    return PSM_E_ARG;
+  }
+  if (nb == 0) return PSM_OK;
+  hipStream_t st = (hipStream_t)stream;
+  if (dtype == PSM_F64)
+    hipLaunchKernelGGL(k_util_mvn_ll<double>, dim3(util_grid(nb)), dim3(kUtilWG), 0, st, nb, n,
+                       (const double*)x, (const double*)L, out, work);
+  else if (dtype == PSM_F32)
+    hipLaunchKernelGGL(k_util_mvn_ll<float>, dim3(util_grid(nb)), dim3(kUtilWG), 0, st, nb, n,
+                       (const float*)x, (const float*)L, out, work);
+  else { set_error("psm_mvn_loglikelihood: bad dtype"); return PSM_E_ARG; }
+  return hip_check("k_util_mvn_ll");
 }
 
 void psm_timing_enable(int32_t enable) {

@@ -41,6 +41,7 @@ EXPORTS = (
     "psm_sample_workspace_bytes", "psm_sample", "psm_seq_filter", "psm_seq_smoother",
     "psm_iterated_smoother_graph", "psm_filter_smoother_noalloc", "psm_adjoint_workspace_bytes",
     "psm_adjoint_prepare", "psm_filter_smoother_vjp", "psm_filter_vjp", "psm_math_check",
+    "psm_tria", "psm_cholesky_update", "psm_mvn_loglikelihood",
 )
 
 
@@ -106,6 +107,9 @@ _SIGS = {
                        + [_P] * 10 + [_P, ctypes.c_size_t, _P]),
     "psm_mean_sq_diff": (ctypes.c_int, [_I32, _I64, _P, _P, _P, _P]),
     "psm_math_check": (ctypes.c_int, [_I64, _P, _P, _P, _P]),
+    "psm_tria": (ctypes.c_int, [_I32, _I64, _I32, _I32, _P, _P]),
+    "psm_cholesky_update": (ctypes.c_int, [_I32, _I64, _I32, _I32, _P, _P, ctypes.c_double, _P, _P]),
+    "psm_mvn_loglikelihood": (ctypes.c_int, [_I32, _I64, _I32, _P, _P, _P, _P, _P]),
     "psm_clear_graphs": (None, []),
     "psm_timing_enable": (None, [_I32]),
     "psm_chunk_size": (ctypes.c_int, [ctypes.POINTER(PsmProblem)]),
