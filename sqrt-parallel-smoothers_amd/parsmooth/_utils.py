@@ -200,3 +200,14 @@ def fixed_point(f, x0, criterion):
         parts.append(out[k:k + leaf.numel()].reshape(leaf.shape))
         k += leaf.numel()
     return rebuild(parts)
+
+
+def _step_arrays(*arrays):
+    """Device tensors of one common dtype for the single-step helpers: float32
+    when every tensor argument is float32, float64 otherwise."""
+    tens = [a for a in arrays if isinstance(a, torch.Tensor)]
+    dt = torch.float32 if tens and all(t.dtype == torch.float32 for t in tens) and \
+        len(tens) == len(arrays) else torch.float64
+    dev = N.device()
+    return [a.to(device=dev, dtype=dt) if isinstance(a, torch.Tensor)
+            else torch.as_tensor(np.asarray(a, dtype=np.float64), device=dev).to(dt) for a in arrays]

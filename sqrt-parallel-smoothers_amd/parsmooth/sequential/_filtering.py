@@ -3,10 +3,13 @@ from __future__ import annotations
 
 from typing import Callable, Union
 
+import torch
+
 from .. import _native as N
 from .._base import MVNSqrt, MVNStandard, are_inputs_compatible
 from .._engine import Models, Problem, compute_dtype, seq_filter_pass, to_dev
 from ..linearization._methods import resolve
+from .._utils import _step_arrays, mvn_loglikelihood, tria
 
 
 def filtering(observations, x0: Union[MVNSqrt, MVNStandard], transition_model, observation_model,
@@ -41,3 +44,49 @@ def filtering(observations, x0: Union[MVNSqrt, MVNStandard], transition_model, o
     if return_loglikelihood:
         return res, ell[0]
     return res
+
+
+# ---------------------------------------------------------------- single steps
+# One prediction or update at a time (parsmooth/sequential/_filtering.py:56-108),
+# for callers and tests that use the steps one by one; the sequential passes
+# run them inside k_seq_filter.  Arguments are device tensors or arrays (moved
+# to the device as float64); the factorisations and log-densities run in the
+# library (psm_tria, psm_mvn_loglikelihood), the small products in torch.
+def _standard_predict(F, Q, b, x):
+    """(F m + b, F P F^T + Q)."""
+    F, Q, b, m, P = _step_arrays(F, Q, b, x[0], x[1])
+    return MVNStandard(F @ m + b, Q + F @ P @ F.T)
+
+
+def _standard_update(H, R, c, x, y):
+    """Kalman update of (m, P) by y: (posterior, log N(y; H m + c, S))."""
+    H, R, c, m, P, y = _step_arrays(H, R, c, x[0], x[1], y)
+    y_diff = y - (H @ m + c)
+    S = R + H @ P @ H.T
+    chol_S = torch.linalg.cholesky(S)
+    G = torch.cholesky_solve(H, chol_S).T
+    G = P @ G
+    ell = mvn_loglikelihood(y_diff, chol_S)
+    return MVNStandard(m + G @ y_diff, P - G @ S @ G.T), ell
+
+
+def _sqrt_predict(F, cholQ, b, x):
+    """(F m + b, tria([F chol P, chol Q]))."""
+    F, cQ, b, m, L = _step_arrays(F, cholQ, b, x[0], x[1])
+    return MVNSqrt(F @ m + b, tria(torch.cat([F @ L, cQ], dim=1)))
+
+
+def _sqrt_update(H, cholR, c, x, y):
+    """Square-root update: tria of [[H chol P, chol R], [chol P, 0]] gives the
+    innovation factor, the gain and the posterior factor."""
+    H, cR, c, m, L, y = _step_arrays(H, cholR, c, x[0], x[1], y)
+    nx, ny = m.shape[0], y.shape[0]
+    y_diff = y - (H @ m + c)
+    M = torch.cat([torch.cat([H @ L, cR], dim=1),
+                   torch.cat([L, torch.zeros(nx, ny, dtype=L.dtype, device=L.device)], dim=1)], dim=0)
+    chol_S = tria(M)
+    G = chol_S[ny:, :ny]
+    I = chol_S[:ny, :ny]
+    step = torch.linalg.solve_triangular(I, y_diff[:, None], upper=False)[:, 0]
+    ell = mvn_loglikelihood(y_diff, I)
+    return MVNSqrt(m + G @ step, chol_S[ny:, ny:].contiguous()), ell
