@@ -1,7 +1,8 @@
 """Parity of the HIP path with the FP64 oracle at the larger compiled sizes
 (state dimension 6-8, observation dimension 4): operators, the linear
 filter/smoother in both forms, every scan schedule family on a ragged batch,
-iterated cubature smoothing of a nonlinear model, FP32, the sequential passes,
+iterated cubature and extended smoothing of a nonlinear model, the
+Gauss-Hermite, unscented and cubature regressions, FP32, the sequential passes,
 the pathwise sampler and the reverse-mode gradient.
 
 The reference takes its sizes from the array shapes
@@ -348,3 +349,52 @@ def test_reverse_gradient_large():
     fd = np.array([(_oracle_grad_ell(theta + h * e, ys) - _oracle_grad_ell(theta - h * e, ys)) / (2 * h)
                    for e in np.eye(3)])
     np.testing.assert_allclose(grad[0].cpu().numpy(), fd, rtol=1e-6, atol=1e-6)
+
+
+@pytest.mark.parametrize("sqrt", [True, False])
+def test_iterated_extended_large(sqrt):
+    """Three extended (IEKS) passes of x' = sin(x) + q at nx = 8 through a
+    linear ny = 4 observation."""
+    from parsmooth.linearization import extended as ext
+    nx, ny, T = 8, 4, 120
+    rng = np.random.RandomState(21)
+    cQ = 0.2 * np.eye(nx) + 0.05 * rand_lower(nx, rng)
+    cR = 0.1 * np.eye(ny) + 0.02 * rand_lower(ny, rng)
+    H = rng.randn(ny, nx) / np.sqrt(nx)
+    x = rng.randn(nx)
+    ys = []
+    for _ in range(T):
+        x = np.sin(x) + cQ @ rng.randn(nx)
+        ys.append(H @ x + cR @ rng.randn(ny))
+    ys = np.asarray(ys)
+    m0, L0 = np.zeros(nx), np.eye(nx)
+    cls_o, cls = (O.MVNSqrt, MVNSqrt) if sqrt else (O.MVNStandard, MVNStandard)
+    q, r = (cQ, cR) if sqrt else (gram(cQ), gram(cR))
+    to = O.FunctionalModel(O.Elementwise("sin"), cls_o(np.zeros(nx), q))
+    oo = O.FunctionalModel(O.Linear(H), cls_o(np.zeros(ny), r))
+    ref = O.iterated_smoothing(ys, cls_o(m0, L0), to, oo, O.extended, None, True, criterion=lambda i, *_: i < 3)
+    td = FunctionalModel(M.Elementwise("sin", nx), cls(t64(np.zeros(nx)), t64(q)))
+    od = FunctionalModel(M.LinearFunction(H), cls(t64(np.zeros(ny)), t64(r)))
+    got = parsmooth.iterated_smoothing(t64(ys), cls(t64(m0), t64(L0)), td, od, ext, criterion=lambda i, *_: i < 3)
+    close(npy(got[0]), ref[0], tol=1e-8, what="mean")
+    close(gram(npy(got[1])) if sqrt else npy(got[1]), gram(ref[1]) if sqrt else ref[1], tol=1e-8, what="cov")
+
+
+@pytest.mark.parametrize("method,nx", [("gauss_hermite", 6), ("unscented", 7), ("cubature", 8)])
+def test_sigma_point_linearize_large(method, nx):
+    """The sigma-point regressions at the larger sizes (Gauss-Hermite order 3:
+    729 points at nx = 6) against the oracle, on an elementwise model."""
+    from parsmooth.linearization import cubature as cub, gauss_hermite, unscented
+    dm = {"gauss_hermite": gauss_hermite, "unscented": unscented, "cubature": cub}[method]
+    om = {"gauss_hermite": O.gauss_hermite, "unscented": O.unscented, "cubature": O.cubature}[method]
+    rng = np.random.RandomState(nx)
+    B = 4
+    m = rng.randn(B, nx)
+    L = np.stack([0.3 * rand_lower(nx, rng) + 0.1 * np.eye(nx) for _ in range(B)])
+    qm, qc = rng.randn(nx), 0.1 * rand_lower(nx, rng) + 0.05 * np.eye(nx)
+    ref = om(O.FunctionalModel(O.Elementwise("arctan"), O.MVNSqrt(qm, qc)), O.MVNSqrt(m, L))
+    got = [npy(g) for g in dm(FunctionalModel(M.Elementwise("arctan", nx), MVNSqrt(t64(qm), t64(qc))),
+                              MVNSqrt(t64(m), t64(L)))]
+    close(got[0], ref[0], tol=1e-8, what=f"{method} F")
+    close(got[2], ref[2], tol=1e-8, what=f"{method} b")
+    close(gram(got[1]), gram(ref[1]), tol=1e-8, what=f"{method} Q")
