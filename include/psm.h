@@ -299,8 +299,12 @@ int psm_iterated_smoother_graph(const psm_problem_t* prob, const psm_model_t* tr
  * parsmooth/_utils.py:118-133 (_fixed_point_rev / _rev_iter: vjp of
  * filter_smoother with respect to the nominal and the model parameters) and
  * jax.grad of the log-likelihood of parsmooth/parallel/_filtering.py:53-60.
- * Square-root form, PSM_F32 / PSM_F64 primal (all adjoint arithmetic and
- * every cotangent array is FP64), models the kernels evaluate.  Cotangents of
+ * Square-root form, PSM_F64 problems (all adjoint arithmetic and every
+ * cotangent array is FP64; other dtypes return PSM_E_UNSUPPORTED and a
+ * workspace size of 0), models the kernels evaluate.  An FP32 objective runs
+ * these on an FP64 problem holding its FP32-rounded observations and model
+ * arrays, as parsmooth.gradient does: the checkpoint of an FP32 primal would
+ * carry FP32 rounding of the smoothed means into every step's residual.  Cotangents of
  * chol factors are lower triangular (the factors' lower triangles are the
  * variables).
  *

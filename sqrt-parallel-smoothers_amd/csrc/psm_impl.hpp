@@ -738,7 +738,10 @@ struct Impl {
   }
 
   // ---------------------------------------------------------------- adjoint
-  static constexpr bool kAdj = SQRT && (std::is_same<R, float>::value || std::is_same<R, double>::value);
+  // FP64 primal only: an FP32 objective's adjoint runs on an FP64 problem that
+  // holds its FP32-rounded arrays (an FP32 checkpoint would carry FP32 rounding
+  // into every step's residual), so no FP32 slice carries adjoint kernels.
+  static constexpr bool kAdj = SQRT && std::is_same<R, double>::value;
   struct AdjPlan {
     int64_t M = 0, Mp = 0;
     size_t off_ck = 0, off_pa = 0, off_lc = 0, off_ln = 0, off_th = 0;
@@ -789,7 +792,8 @@ struct Impl {
   }
   static int adj_check(const psm_problem_t* p, const psm_model_t* tr, const psm_model_t* ob, size_t awsb) {
     if constexpr (!kAdj) {
-      set_error("the adjoint supports the square-root form in PSM_F32 / PSM_F64 only");
+      set_error("the adjoint supports the square-root form in PSM_F64 only (run an FP32 objective's "
+                "adjoint on an FP64 problem holding its FP32-rounded arrays)");
       return PSM_E_UNSUPPORTED;
     } else {
       for (const psm_model_t* m : {tr, ob}) {

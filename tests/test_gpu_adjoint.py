@@ -130,3 +130,26 @@ def test_reverse_gradient_matches_forward_mode(method, dtype):
     print(f"{method.__name__} {dtype}: reverse vs forward gradient rel err {err:.2e}", flush=True)
     assert torch.equal(ell_r, ell_f)
     assert err < (1e-8 if dtype == torch.float64 else 1e-4), (g_r, g_f)
+
+
+def test_adjoint_rejects_fp32_problems():
+    """The adjoint ABI takes FP64 problems only: an FP32 problem gets a
+    workspace size of 0 and PSM_E_UNSUPPORTED, and FP32 objectives reach the
+    adjoint through their FP64 twin (test_reverse_gradient_matches_forward_mode)."""
+    import ctypes
+    from parsmooth.gradient import _ModelSet
+    obj, theta, init = _objective(extended, dtype=torch.float32)
+    assert int(N.lib().psm_adjoint_workspace_bytes(ctypes.byref(obj.prob.struct))) == 0
+    vals = obj.build(torch.as_tensor(theta))
+    mv = _ModelSet(obj.kinds, (5, 2), vals, DEV, torch.float32)
+    nm = (torch.as_tensor(init[0], dtype=torch.float32, device=DEV).contiguous(),
+          torch.as_tensor(init[1], dtype=torch.float32, device=DEV).contiguous())
+    f, s = obj._new(False), obj._new(False)
+    aws = torch.empty(1 << 20, dtype=torch.uint8, device=DEV)
+    rc = N.lib().psm_adjoint_prepare(ctypes.byref(obj.prob.struct), ctypes.byref(mv.trans), ctypes.byref(mv.obs),
+                                     None, N.ptr(obj.ys), N.ptr(mv.keep["m0"]), N.ptr(mv.keep["L0"]),
+                                     N.ptr(nm[0]), N.ptr(nm[1]), N.ptr(f[0]), N.ptr(f[1]), N.ptr(s[0]),
+                                     N.ptr(s[1]), N.ptr(obj.prob.workspace), obj.prob.ws_bytes, N.ptr(aws),
+                                     aws.numel(), N.stream_ptr())
+    assert rc != 0
+    assert b"PSM_F64" in N.lib().psm_last_error()
