@@ -96,3 +96,21 @@ def test_none_or_shift_and_concat():
     assert torch.equal(back.chol[-1], x0.chol) and torch.equal(back.chol[:-1], traj.chol[:-1])
     arr = none_or_concat(np.zeros((2, 2)), np.ones(2), 1)
     assert arr.shape == (3, 2) and np.all(arr[0] == 1.0)
+
+
+def test_reference_module_paths():
+    """The module paths the reference's tests and users import resolve to this
+    package's functions."""
+    from parsmooth.linearization import cubature, extended, gauss_hermite, unscented
+    from parsmooth.linearization._cubature import linearize as c
+    from parsmooth.linearization._extended import linearize as e
+    from parsmooth.linearization._gh import linearize as g
+    from parsmooth.linearization._unscented import linearize as u
+    assert (c, e, g, u) == (cubature, extended, gauss_hermite, unscented)
+    from parsmooth._utils import _cholesky_update, cholesky_update_many, mvn_loglikelihood, tria  # noqa: F401
+    from parsmooth.parallel._filtering import _sqrt_associative_params_one, _standard_associative_params_one  # noqa: F401
+    from parsmooth.parallel._operators import sqrt_filtering_operator, sqrt_smoothing_operator  # noqa: F401
+    from parsmooth.parallel._smoothing import _sqrt_associative_params, _standard_associative_params  # noqa: F401
+    from parsmooth.sequential._filtering import _sqrt_predict, _sqrt_update, _standard_predict, _standard_update  # noqa: F401
+    from parsmooth.sequential._smoothing import _sqrt_smooth, _standard_smooth  # noqa: F401
+    from parsmooth.methods import iterated_smoothing, sampling  # noqa: F401
