@@ -1,5 +1,6 @@
 """parsmooth._utils host helpers (CPU): fixed_point and its implicit gradient
-(tests/test_math_utils.py:55-66 of the reference), the trajectory shifts."""
+(tests/test_math_utils.py:55-66 of the reference), the trajectory shifts, and
+the reference's module paths."""
 import numpy as np
 import pytest
 import torch
