@@ -34,8 +34,8 @@ struct Step {
   R H[NY][NX];
   R cr[NY][NY];   // chol R (sqrt form) or R
   R c[NY];
-  static constexpr int NTRANS = 2 * NX * NX + NX;
-  static constexpr int NOBS = NY * NX + NY * NY + NY;
+  static constexpr int NTRANS = NX * NX + NX * (NX + 1) / 2 + NX;
+  static constexpr int NOBS = NY * NX + NY * (NY + 1) / 2 + NY;
   static constexpr int NQ = NTRANS + NOBS;
 };
 
@@ -86,18 +86,39 @@ PSM_DEV void st(R* __restrict__ base, int64_t stride, int64_t i, int& q, const R
     for (int b = 0; b < N; ++b) base[(int64_t)(q++) * stride + i] = v[a][b];
 }
 
-template <typename R, int NX, int NY>
+// Lower triangle (row by row) of an N x N noise factor or covariance; on load
+// the upper triangle is zero (factor) or the mirror image (SYM: covariance).
+template <typename R, int N>
+PSM_DEV void st_tri(R* __restrict__ base, int64_t stride, int64_t i, int& q, const R (&v)[N][N]) {
+#pragma unroll
+  for (int a = 0; a < N; ++a)
+#pragma unroll
+    for (int b = 0; b <= a; ++b) base[(int64_t)(q++) * stride + i] = v[a][b];
+}
+template <bool SYM, typename R, int N>
+PSM_DEV void ld_tri(const R* __restrict__ base, int64_t stride, int64_t i, int& q, R (&v)[N][N]) {
+#pragma unroll
+  for (int a = 0; a < N; ++a)
+#pragma unroll
+    for (int b = 0; b <= a; ++b) v[a][b] = base[(int64_t)(q++) * stride + i];
+#pragma unroll
+  for (int a = 0; a < N; ++a)
+#pragma unroll
+    for (int b = a + 1; b < N; ++b) v[a][b] = SYM ? v[b][a] : R(0);
+}
+
+template <bool SYM, typename R, int NX, int NY>
 PSM_DEV void load_trans(const R* base, int64_t stride, int64_t i, Step<R, NX, NY>& s) {
   int q = 0;
   ld(base, stride, i, q, s.F);
-  ld(base, stride, i, q, s.cq);
+  ld_tri<SYM>(base, stride, i, q, s.cq);
   ld(base, stride, i, q, s.bt);
 }
-template <typename R, int NX, int NY>
+template <bool SYM, typename R, int NX, int NY>
 PSM_DEV void load_obs(const R* base, int64_t stride, int64_t i, Step<R, NX, NY>& s) {
   int q = Step<R, NX, NY>::NTRANS;
   ld(base, stride, i, q, s.H);
-  ld(base, stride, i, q, s.cr);
+  ld_tri<SYM>(base, stride, i, q, s.cr);
   ld(base, stride, i, q, s.c);
 }
 

@@ -307,7 +307,8 @@ __global__ __launch_bounds__(kLaneWG) void k_linearize_states(
     linearize_point_w<R, NX, NX, SQRT>(tr, sp, use_sigma, m, L, F, Cq, b, fb, pc);
     int q = 0;
     st(lin, Mp, t, q, F);
-    st(lin, Mp, t, q, Cq);
+    if constexpr (SQRT) lower_factor(Cq);
+    st_tri(lin, Mp, t, q, Cq);
     st(lin, Mp, t, q, b);
   }
   if (do_obs) {
@@ -315,7 +316,8 @@ __global__ __launch_bounds__(kLaneWG) void k_linearize_states(
     linearize_point_w<R, NX, NY, SQRT>(ob, sp, use_sigma, m, L, H, Cr, c, fb);
     int q = Step<R, NX, NY>::NTRANS;
     st(lin, Mp, t, q, H);
-    st(lin, Mp, t, q, Cr);
+    if constexpr (SQRT) lower_factor(Cr);
+    st_tri(lin, Mp, t, q, Cr);
     st(lin, Mp, t, q, c);
   }
 }
@@ -348,8 +350,8 @@ __global__ __launch_bounds__(kLaneWG) void k_filter_reduce(
       a.flag = FLAG_HEAD;
     } else {
       Step<R, NX, NY> st_;
-      load_trans(lin, Mp, sm(i - 1), st_);
-      load_obs(lin, Mp, sm(i), st_);
+      load_trans<!SQRT>(lin, Mp, sm(i - 1), st_);
+      load_obs<!SQRT>(lin, Mp, sm(i), st_);
       R y[NY];
       ld_aos(ys, i - s - 1, y);
       absorb_fb<R, NX, NY, SQRT>(a, st_, y, fb);
@@ -692,8 +694,8 @@ __global__ __launch_bounds__(kLaneWG) void k_filter_apply(
       ld_aos(L0, s, L);
     } else {
       Step<R, NX, NY> st_;
-      load_trans(lin, Mp, sm(i - 1), st_);
-      load_obs(lin, Mp, sm(i), st_);
+      load_trans<!SQRT>(lin, Mp, sm(i - 1), st_);
+      load_obs<!SQRT>(lin, Mp, sm(i), st_);
       R y[NY];
       ld_aos(ys, i - s - 1, y);
       e = kf_step_fb<R, NX, NY, SQRT>(st_, y, m, L, fb);
@@ -754,7 +756,7 @@ __global__ __launch_bounds__(kLaneWG) void k_smoother_reduce(
       int q = 0;
       const int64_t pi = sm(i);
       ld(lin, Mp, pi, q, F);
-      ld(lin, Mp, pi, q, cq);
+      ld_tri<!SQRT>(lin, Mp, pi, q, cq);
       ld(lin, Mp, pi, q, bt);
       smoother_element_fb<R, NX, SQRT>(F, cq, bt, m, L, e, fb);
     }

@@ -194,6 +194,20 @@ PSM_DEV void tria_inplace(R (&M)[ROWS][COLS]) {
   }
 }
 
+// A lower-triangular square root of C C^T: C itself when it is lower
+// triangular, else tria(C).  The linearized-model layout keeps only the lower
+// triangle of a noise factor, and a caller's factor (the extended
+// linearization passes chol Q through) may be any square root.
+template <typename R, int N>
+PSM_DEV void lower_factor(R (&C)[N][N]) {
+  bool upper = false;
+#pragma unroll
+  for (int a = 0; a < N; ++a)
+#pragma unroll
+    for (int b = a + 1; b < N; ++b) upper = upper || (C[a][b] != R(0));
+  if (upper) tria_inplace<R, N, N>(C);
+}
+
 // L (ROWS x ROWS) = tria(M)
 template <typename R, int ROWS, int COLS>
 PSM_DEV void tria(R (&M)[ROWS][COLS], R (&L)[ROWS][ROWS]) {

@@ -398,3 +398,31 @@ def test_sigma_point_linearize_large(method, nx):
     close(got[0], ref[0], tol=1e-8, what=f"{method} F")
     close(got[2], ref[2], tol=1e-8, what=f"{method} b")
     close(gram(got[1]), gram(ref[1]), tol=1e-8, what=f"{method} Q")
+
+
+@pytest.mark.parametrize("nx,ny", [(4, 2), (8, 4)])
+def test_noise_factor_any_square_root(nx, ny):
+    """Noise factors given as any square root of Q and R (a lower factor times an
+    orthogonal matrix, so with entries above the diagonal): the linearized
+    layout keeps lower factors, so the kernels triangularise such a factor
+    first; the result matches the oracle, which takes any square root."""
+    T = 50
+    x0, cx0, F, Q, cholQ, b, H, R, cholR, c, ys = _linear_problem(nx, ny, 8, T)
+    rng = np.random.RandomState(9)
+    sQ = cholQ @ np.linalg.qr(rng.randn(nx, nx))[0]
+    sR = cholR @ np.linalg.qr(rng.randn(ny, ny))[0]
+    assert np.abs(np.triu(sQ, 1)).max() > 0.1 and np.allclose(sQ @ sQ.T, Q)
+    to = O.FunctionalModel(O.Linear(F), O.MVNSqrt(b, sQ))
+    oo = O.FunctionalModel(O.Linear(H), O.MVNSqrt(c, sR))
+    fo, ell_o = O.filtering(ys, cx0, to, oo, O.extended, None, True, return_loglikelihood=True)
+    so = O.smoothing(to, fo, O.extended, None, True)
+    td = FunctionalModel(M.LinearFunction(F), MVNSqrt(t64(b), t64(sQ)))
+    od = FunctionalModel(M.LinearFunction(H), MVNSqrt(t64(c), t64(sR)))
+    fd, ell_d = parsmooth.filtering(t64(ys), MVNSqrt(t64(cx0.mean), t64(cx0.chol)), td, od, extended,
+                                    return_loglikelihood=True)
+    close(npy(fd.mean), fo.mean, what="filtered mean")
+    close(gram(npy(fd.chol)), gram(fo.chol), what="filtered cov")
+    assert abs(float(ell_d) - ell_o) <= TOL64 * max(1.0, abs(ell_o))
+    sd = parsmooth.smoothing(td, fd, extended)
+    close(npy(sd.mean), so.mean, what="smoothed mean")
+    close(gram(npy(sd.chol)), gram(so.chol), what="smoothed cov")
