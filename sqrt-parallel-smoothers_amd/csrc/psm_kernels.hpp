@@ -130,27 +130,31 @@ PSM_DEV void st_aos(R* __restrict__ p, int64_t i, const R (&v)[M][N]) {
 // array-of-structs (state i at m + i NX, L + i NX^2; inter == 0), or, inside a
 // launch sequence that owns them, the state-interleaved layout of the
 // workspace (inter != 0: component q of state i at m[q Mp + pos], pos its
-// interleaved position, means first), which lane-per-chunk kernels read and
+// interleaved position, means first, then a factor's lower triangle or a
+// covariance in full (SYM), so that a launch sequence and the host loop over
+// the same passes agree bit for bit), which lane-per-chunk kernels read and
 // write coalesced.
-template <typename R, int NX>
+template <bool SYM, typename R, int NX>
 PSM_DEV void traj_load(const R* __restrict__ m_, const R* __restrict__ L_, int inter, int64_t Mp,
                        int64_t i, int64_t pos, R (&m)[NX], R (&L)[NX][NX]) {
   if (inter) {
     int q = 0;
     ld(m_, Mp, pos, q, m);
-    ld(m_, Mp, pos, q, L);
+    if constexpr (SYM) ld(m_, Mp, pos, q, L);
+    else ld_tri<false>(m_, Mp, pos, q, L);
   } else {
     ld_aos(m_, i, m);
     ld_aos(L_, i, L);
   }
 }
-template <typename R, int NX>
+template <bool SYM, typename R, int NX>
 PSM_DEV void traj_store(R* __restrict__ m_, R* __restrict__ L_, int inter, int64_t Mp, int64_t i,
                         int64_t pos, const R (&m)[NX], const R (&L)[NX][NX]) {
   if (inter) {
     int q = 0;
     st(m_, Mp, pos, q, m);
-    st(m_, Mp, pos, q, L);
+    if constexpr (SYM) st(m_, Mp, pos, q, L);
+    else st_tri(m_, Mp, pos, q, L);
   } else {
     st_aos(m_, i, m);
     st_aos(L_, i, L);
@@ -295,7 +299,7 @@ __global__ __launch_bounds__(kLaneWG) void k_linearize_states(
   tr.row = i;
   ob.row = i;
   R m[NX], L[NX][NX];
-  traj_load<R, NX>(nom_m, nom_L, nom_inter, Mp, i, t, m, L);
+  traj_load<!SQRT, R, NX>(nom_m, nom_L, nom_inter, Mp, i, t, m, L);
   // FP32: the regression's increments wait in LDS for its residual pass
   PointCache<R> pc{nullptr, 0, 0};
   if constexpr (std::is_same<R, float>::value) {
@@ -700,7 +704,7 @@ __global__ __launch_bounds__(kLaneWG) void k_filter_apply(
       ld_aos(ys, i - s - 1, y);
       e = kf_step_fb<R, NX, NY, SQRT>(st_, y, m, L, fb);
     }
-    traj_store<R, NX>(out_m, out_L, out_inter, Mp, i, sm(i), m, L);
+    traj_store<!SQRT, R, NX>(out_m, out_L, out_inter, Mp, i, sm(i), m, L);
     if (ell_state != nullptr) ell_state[i] = e;
   }
 }
@@ -744,7 +748,7 @@ __global__ __launch_bounds__(kLaneWG) void k_smoother_reduce(
   for (int64_t i = i1 - 1; i >= i0; --i) {
     while (i < soff[s]) --s;
     R m[NX], L[NX][NX];
-    traj_load<R, NX>(fm, fL, f_inter, Mp, i, sm(i), m, L);
+    traj_load<!SQRT, R, NX>(fm, fL, f_inter, Mp, i, sm(i), m, L);
     SAgg<R, NX> e;
     if (i == soff[s + 1] - 1) {
       copy(m, e.g);
@@ -796,7 +800,7 @@ __global__ __launch_bounds__(kLaneWG) void k_smoother_apply(
     } else {
       smoother_apply_fb<R, NX, SQRT>(e, m, D, fb);
     }
-    traj_store<R, NX>(out_m, out_L, out_inter, Mp, i, sm(i), m, D);
+    traj_store<!SQRT, R, NX>(out_m, out_L, out_inter, Mp, i, sm(i), m, D);
   }
 }
 
